@@ -1,0 +1,191 @@
+/*
+ * kami.h -- C ABI of the MI355X (gfx950) block-level GEMM library.
+ *
+ * Three entry-point families, all device-pointer based and caller-owned:
+ *   block GEMM      kami_block_gemm            one problem per workgroup, fixed shape
+ *   batched GEMM    kami_batched_plan_*        uneven shapes, persistent kernel + queue
+ *   block SpGEMM    kami_bsr_spgemm_*          symbolic (count/scan/fill) + numeric
+ *
+ * Conventions shared by every entry point:
+ *   - Matrices are row-major. A is M x K. B is N x K ("B transposed", the layout of
+ *     the reference's mma.row.col kernels) unless KAMI_B_KN is selected, in which
+ *     case B is K x N. C is M x N. Leading dimensions are in elements, 0 = packed.
+ *   - Every pointer argument that names matrix data is a device pointer. The library
+ *     never frees caller memory and never allocates in kami_block_gemm.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream). Every
+ *     launch is asynchronous on that stream; the only synchronising call is
+ *     kami_bsr_spgemm_symbolic, which reads the output block count back to the host.
+ *   - Errors are returned as kami_status; the library prints nothing.
+ *
+ * Reference interfaces replaced (paths relative to the KAMI source tree):
+ *   kami_block_gemm   <- block_gemm_{1d,2d,3d}_{double,half}_mma kernels + their
+ *                        launch in main(): src/block_gemm/KAMI/1d/fp64/
+ *                        block_gemm_1d_double_mma.cu:55 and :263, 2d/.../:61 and :281,
+ *                        3d/.../:62 and :294 (+ host slab sum :324-333), fp16 variants
+ *                        src/block_gemm/KAMI/1d/fp16/block_gemm_1d_half_mma.cu:61.
+ *   kami_batched_*    <- batched_gemm_{1d,2d,3d}_double_mma: src/batched_gemm/KAMI/1d/
+ *                        batched_gemm_1d_double_mma.cu:59 and its launch :273.
+ *   kami_bsr_spgemm_* <- spgemm_symbolic_kernel + thrust::exclusive_scan +
+ *                        spgemm_symbolic_col_kernel + block_spgemm_*_half_mma:
+ *                        src/sparse/spgemm/2d/block_spgemm_2d_half_mma.cu:58, :806,
+ *                        :88, :171 and the host sequence :797-825.
+ */
+#ifndef KAMI_H
+#define KAMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KAMI_VERSION_MAJOR 0
+#define KAMI_VERSION_MINOR 1
+
+typedef enum kami_status {
+    KAMI_OK = 0,
+    KAMI_EINVAL = 1,  /* bad argument: null pointer, bad shape, stride, alignment */
+    KAMI_ENOTSUP = 2, /* valid request with no compiled instance */
+    KAMI_ENOMEM = 3,  /* host or device allocation failed */
+    KAMI_EHIP = 4,    /* a HIP runtime call failed, see kami_last_hip_error() */
+    KAMI_ENOSPC = 5,  /* caller workspace too small */
+    KAMI_ENODEV = 6   /* no gfx950 device is current */
+} kami_status;
+
+typedef enum kami_precision {
+    KAMI_PREC_F64 = 0,            /* A,B,C double; native FP64 MFMA */
+    KAMI_PREC_F16_F32 = 1,        /* A,B IEEE binary16; C float; FP32 accumulation */
+    KAMI_PREC_BF16_F32 = 2,       /* A,B bfloat16; C float; FP32 accumulation */
+    KAMI_PREC_F64_SPLIT_F16 = 3,  /* A,B,C double; FP16 slices on MFMA, FP64 recombination */
+    KAMI_PREC_F64_SPLIT_BF16 = 4  /* A,B,C double; BF16 slices on MFMA, FP64 recombination */
+} kami_precision;
+
+typedef enum kami_partition {
+    KAMI_PART_AUTO = 0, /* tuned choice */
+    KAMI_PART_1D = 1,   /* waves split M; B broadcast through LDS; A register-resident */
+    KAMI_PART_2D = 2,   /* waves on an M x N grid; A and B broadcast along rows/columns */
+    KAMI_PART_3D = 3    /* waves on an M x N x K grid; K partial sums reduced in LDS */
+} kami_partition;
+
+typedef enum kami_b_layout {
+    KAMI_B_NK = 0, /* B stored N x K row-major (reference layout, C = A * B^T of storage) */
+    KAMI_B_KN = 1  /* B stored K x N row-major */
+} kami_b_layout;
+
+/* Descriptor of one block-GEMM shape. Fill with kami_gemm_desc_init() then edit. */
+typedef struct kami_gemm_desc {
+    int32_t m, n, k;
+    int32_t precision; /* kami_precision */
+    int32_t partition; /* kami_partition */
+    int32_t ranks;     /* wave-grid edge P (1D: P waves, 2D: P*P, 3D: P*P*P); 0 = default */
+    int32_t b_layout;  /* kami_b_layout */
+    int32_t iters;     /* >= 1: the product is accumulated `iters` times before the
+                          epilogue, as the reference's NUM_ITER repeat does; 1 = plain GEMM */
+    int64_t lda, ldb, ldc; /* elements; 0 = packed */
+    double alpha, beta;    /* C = alpha * (A B) + beta * C */
+    int32_t slices;        /* split precision: slices per operand, 0 = default */
+    int32_t reserved;
+} kami_gemm_desc;
+
+/* The instance a descriptor resolves to (filled by kami_block_gemm_query). */
+typedef struct kami_gemm_config {
+    int32_t partition;      /* resolved kami_partition */
+    int32_t ranks;          /* resolved P */
+    int32_t waves_m, waves_n, waves_k; /* wave grid */
+    int32_t threads;        /* workgroup size */
+    int32_t k_stage;        /* K elements staged through LDS per stage */
+    int32_t lds_bytes;      /* static LDS of the instance */
+    int32_t slices;         /* split precision slices (0 otherwise) */
+    int32_t instance;       /* internal instance id */
+} kami_gemm_config;
+
+/* One problem of a batched plan. Pointers are device pointers that must stay valid
+ * for the plan's lifetime. */
+typedef struct kami_problem {
+    int32_t m, n, k, reserved;
+    int64_t lda, ldb, ldc; /* elements; 0 = packed */
+    const void* A;
+    const void* B;
+    void* C;
+} kami_problem;
+
+typedef struct kami_batched_opts {
+    int32_t b_layout;  /* kami_b_layout */
+    int32_t autotune;  /* 1 = time the candidate tile variants on the device at plan
+                          creation; 0 = use the built-in priors */
+    int32_t use_graph; /* 1 = execute through a captured hipGraph */
+    int32_t reserved;
+    double alpha, beta;
+} kami_batched_opts;
+
+typedef struct kami_batched_plan kami_batched_plan;
+
+typedef struct kami_plan_info {
+    int64_t problems;
+    int64_t work_items;     /* queue entries (output tiles) of one execution */
+    int32_t buckets;        /* size buckets the autotuner decided */
+    int32_t workgroups;     /* persistent grid size */
+    double flops;           /* 2*sum(M*N*K) of one execution */
+    int32_t variant_of_bucket[16]; /* chosen tile variant per bucket (first 16) */
+} kami_plan_info;
+
+/* Block-sparse row matrix: mb x nb blocks of bs x bs, blocks row-major. */
+typedef struct kami_bsr {
+    int32_t mb, nb, bs, nnzb;
+    const int32_t* rowptr; /* device, mb + 1 */
+    const int32_t* colidx; /* device, nnzb, ascending within each block row */
+    const void* vals;      /* device, nnzb * bs * bs elements */
+} kami_bsr;
+
+/* ---------------------------------------------------------------- status, device */
+const char* kami_status_string(kami_status s);
+int32_t kami_last_hip_error(void);       /* hipError_t of the last failing HIP call on this thread */
+const char* kami_version(void);
+kami_status kami_device_check(void);     /* KAMI_OK when the current device is gfx950 */
+
+/* ---------------------------------------------------------------- block GEMM */
+kami_status kami_gemm_desc_init(kami_gemm_desc* d, int32_t m, int32_t n, int32_t k,
+                                int32_t precision);
+/* Validate a descriptor and report the instance it resolves to. Touches no device. */
+kami_status kami_block_gemm_query(const kami_gemm_desc* d, kami_gemm_config* out);
+/* count independent problems; problem p uses A + p*stride_a, B + p*stride_b,
+ * C + p*stride_c (elements; 0 = packed: m*k, n*k, m*n). One problem per workgroup. */
+kami_status kami_block_gemm(const kami_gemm_desc* d, const void* A, const void* B, void* C,
+                            int64_t count, int64_t stride_a, int64_t stride_b,
+                            int64_t stride_c, void* stream);
+
+/* ---------------------------------------------------------------- batched GEMM */
+/* precision: KAMI_PREC_F64 or KAMI_PREC_F16_F32 or KAMI_PREC_BF16_F32.
+ * Builds the work queue, autotunes per size bucket (runs kernels on `stream` when
+ * opts->autotune is set: C buffers are written during tuning). */
+kami_status kami_batched_plan_create(const kami_problem* probs, int64_t count,
+                                     int32_t precision, const kami_batched_opts* opts,
+                                     void* stream, kami_batched_plan** out);
+/* One persistent launch (through the plan's hipGraph when use_graph is set). */
+kami_status kami_batched_execute(kami_batched_plan* plan, void* stream);
+kami_status kami_batched_plan_info(const kami_batched_plan* plan, kami_plan_info* out);
+kami_status kami_batched_plan_destroy(kami_batched_plan* plan);
+
+/* ---------------------------------------------------------------- block SpGEMM */
+/* Workspace for symbolic/fill of C = A * B, A with mb block rows, B with nb block cols. */
+kami_status kami_bsr_spgemm_workspace_size(int32_t mb, int32_t nb, size_t* bytes);
+/* rowptrC (mb+1, device) receives the exclusive scan of per-row output block counts;
+ * *nnzbC receives rowptrC[mb] (copied to the host: this call synchronises `stream`). */
+kami_status kami_bsr_spgemm_symbolic(const kami_bsr* A, const kami_bsr* B, int32_t* rowptrC,
+                                     int64_t* nnzbC, void* ws, size_t ws_bytes, void* stream);
+/* colidxC (nnzbC, device) receives the ascending output block columns of every row. */
+kami_status kami_bsr_spgemm_fill(const kami_bsr* A, const kami_bsr* B, const int32_t* rowptrC,
+                                 int32_t* colidxC, void* ws, size_t ws_bytes, void* stream);
+/* valC (nnzbC * bs * bs, device) = alpha * (A * B) on the structure of rowptrC/colidxC.
+ * precision: KAMI_PREC_F16_F32 or KAMI_PREC_BF16_F32 (FP16/BF16 values, float C) or
+ * KAMI_PREC_F64 (double values and C). */
+kami_status kami_bsr_spgemm_numeric(const kami_bsr* A, const kami_bsr* B, const int32_t* rowptrC,
+                                    const int32_t* colidxC, int64_t nnzbC, void* valC,
+                                    int32_t precision, double alpha, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KAMI_H */

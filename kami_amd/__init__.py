@@ -1,0 +1,117 @@
+"""kami_amd -- MI355X (gfx950) block-level GEMM, batched GEMM and block SpGEMM.
+
+Python host side over the C ABI of ``libkami.so`` (``include/kami.h``).  Device
+memory and streams come from PyTorch-ROCm tensors; every computation runs in the
+library's HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+
+from . import _abi
+from ._abi import (B_KN, B_NK, PART_1D, PART_2D, PART_3D, PART_AUTO, PREC_BF16_F32,
+                   PREC_F16_F32, PREC_F64, PREC_F64_SPLIT_BF16, PREC_F64_SPLIT_F16,
+                   KamiError, check, lib)
+
+__all__ = [
+    "block_gemm", "query", "make_desc", "KamiError", "lib", "version", "device_ok",
+    "PREC_F64", "PREC_F16_F32", "PREC_BF16_F32", "PREC_F64_SPLIT_F16", "PREC_F64_SPLIT_BF16",
+    "PART_AUTO", "PART_1D", "PART_2D", "PART_3D", "B_NK", "B_KN",
+]
+
+_PART_NAMES = {"auto": PART_AUTO, "1d": PART_1D, "2d": PART_2D, "3d": PART_3D}
+_LAYOUT_NAMES = {"nk": B_NK, "kn": B_KN}
+
+
+def _part(p):
+    return _PART_NAMES[p.lower()] if isinstance(p, str) else int(p)
+
+
+def _layout(b):
+    return _LAYOUT_NAMES[b.lower()] if isinstance(b, str) else int(b)
+
+
+def version() -> str:
+    return lib().kami_version().decode()
+
+
+def device_ok() -> bool:
+    return lib().kami_device_check() == _abi.OK
+
+
+def make_desc(m, n, k, precision, *, partition="auto", ranks=0, b_layout="nk", iters=1,
+              alpha=1.0, beta=0.0, lda=0, ldb=0, ldc=0, slices=0) -> _abi.GemmDesc:
+    d = _abi.GemmDesc()
+    check(lib().kami_gemm_desc_init(ctypes.byref(d), m, n, k, precision), "kami_gemm_desc_init")
+    d.partition = _part(partition)
+    d.ranks = ranks
+    d.b_layout = _layout(b_layout)
+    d.iters = iters
+    d.alpha = alpha
+    d.beta = beta
+    d.lda, d.ldb, d.ldc = lda, ldb, ldc
+    d.slices = slices
+    return d
+
+
+def query(m, n, k, precision, **kw) -> dict:
+    """The compiled instance a descriptor resolves to (no device access)."""
+    d = make_desc(m, n, k, precision, **kw)
+    cfg = _abi.GemmConfig()
+    check(lib().kami_block_gemm_query(ctypes.byref(d), ctypes.byref(cfg)), "kami_block_gemm_query")
+    return cfg.as_dict()
+
+
+def _precision_of(A, precision):
+    import torch
+    if precision is not None:
+        return precision
+    if A.dtype == torch.float64:
+        return PREC_F64
+    if A.dtype == torch.float16:
+        return PREC_F16_F32
+    if A.dtype == torch.bfloat16:
+        return PREC_BF16_F32
+    raise TypeError(f"unsupported input dtype {A.dtype}")
+
+
+def _stream_ptr(stream):
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def block_gemm(A, B, C=None, *, precision=None, partition="auto", ranks=0, b_layout="nk",
+               iters=1, alpha=1.0, beta=0.0, slices=0, stream=None):
+    """C = alpha * A @ op(B) + beta * C for a batch of problems, one per workgroup.
+
+    A: (count, M, K) or (M, K); B: (count, N, K) ("nk", the reference layout) or
+    (count, K, N) ("kn"); C: (count, M, N), float64 for FP64 and split precision,
+    float32 for FP16/BF16 inputs.  Tensors must be contiguous device tensors.
+    """
+    import torch
+    squeeze = A.dim() == 2
+    if squeeze:
+        A, B = A.unsqueeze(0), B.unsqueeze(0)
+        if C is not None:
+            C = C.unsqueeze(0)
+    count, M, K = A.shape
+    lay = _layout(b_layout)
+    N = B.shape[1] if lay == B_NK else B.shape[2]
+    prec = _precision_of(A, precision)
+    out_dtype = torch.float32 if prec in (PREC_F16_F32, PREC_BF16_F32) else torch.float64
+    if C is None:
+        C = torch.empty((count, M, N), dtype=out_dtype, device=A.device)
+    for t in (A, B, C):
+        if not t.is_contiguous():
+            raise ValueError("block_gemm expects contiguous tensors")
+    if C.dtype != out_dtype or tuple(C.shape) != (count, M, N):
+        raise ValueError(f"C must be {out_dtype} of shape {(count, M, N)}")
+    d = make_desc(M, N, K, prec, partition=partition, ranks=ranks, b_layout=lay, iters=iters,
+                  alpha=alpha, beta=beta, slices=slices)
+    st = lib().kami_block_gemm(ctypes.byref(d), ctypes.c_void_p(A.data_ptr()),
+                               ctypes.c_void_p(B.data_ptr()), ctypes.c_void_p(C.data_ptr()),
+                               count, 0, 0, 0, _stream_ptr(stream))
+    check(st, f"kami_block_gemm({M}x{N}x{K}, prec={prec}, part={partition}, ranks={ranks})")
+    return C[0] if squeeze else C

@@ -1,0 +1,50 @@
+// Block-GEMM instances: FP16 inputs, FP32 accumulation (16x16x32 and 16x16x16 f16 MFMA).
+// Shapes are the reference's FP16 sweep (scripts/fp16_block_square_GH200.sh:11-25,
+// scripts/fp16_block_lowrank_GH200.sh:20-27) restricted to the square sizes
+// {32, 64, 128, 256} and the low-rank 256 x 256 x {16, 32}.  A 3D partition at
+// M = N = 256 is not offered: its per-wave FP32 accumulator tile exceeds the register
+// file at every legal workgroup size.
+#include "block_gemm_registry.hpp"
+
+namespace kami {
+
+void register_f16_instances(std::vector<BlockInstance>& v) {
+    using A = AtomF16x32;
+    using H = AtomF16x16;  // K step 16, for K slices that are not multiples of 32
+    constexpr int F = 1;
+    // 32^3
+    add_instance<A, F, 32, 32, 32, 1, 1, 1>(v, 1, 1);
+    add_instance<A, F, 32, 32, 32, 2, 1, 1>(v, 1, 2);
+    add_instance<A, F, 32, 32, 32, 1, 1, 1>(v, 2, 1);
+    add_instance<A, F, 32, 32, 32, 2, 2, 1>(v, 2, 2);
+    add_instance<A, F, 32, 32, 32, 1, 1, 1>(v, 3, 1);
+    add_instance<H, F, 32, 32, 32, 2, 2, 2>(v, 3, 2);
+    // 64^3
+    add_instance<A, F, 64, 64, 64, 1, 1, 1>(v, 1, 1);
+    add_instance<A, F, 64, 64, 64, 2, 1, 1>(v, 1, 2);
+    add_instance<A, F, 64, 64, 64, 4, 1, 1>(v, 1, 4);
+    add_instance<A, F, 64, 64, 64, 1, 1, 1>(v, 2, 1);
+    add_instance<A, F, 64, 64, 64, 2, 2, 1>(v, 2, 2);
+    add_instance<A, F, 64, 64, 64, 4, 4, 1>(v, 2, 4);
+    add_instance<A, F, 64, 64, 64, 1, 1, 1>(v, 3, 1);
+    add_instance<A, F, 64, 64, 64, 2, 2, 2>(v, 3, 2);
+    // 128^3
+    add_instance<A, F, 128, 128, 128, 4, 1, 1>(v, 1, 4);
+    add_instance<A, F, 128, 128, 128, 8, 1, 1>(v, 1, 8);
+    add_instance<A, F, 128, 128, 128, 2, 2, 1>(v, 2, 2);
+    add_instance<A, F, 128, 128, 128, 4, 4, 1>(v, 2, 4);
+    add_instance<A, F, 128, 128, 128, 2, 2, 2>(v, 3, 2);
+    // 256^3
+    add_instance<A, F, 256, 256, 256, 8, 1, 1>(v, 1, 8);
+    add_instance<A, F, 256, 256, 256, 16, 1, 1>(v, 1, 16);
+    add_instance<A, F, 256, 256, 256, 4, 4, 1>(v, 2, 4);
+    // low rank 256 x 256 x 16 and x 32
+    add_instance<H, F, 256, 256, 16, 8, 1, 1>(v, 1, 8);
+    add_instance<H, F, 256, 256, 16, 16, 1, 1>(v, 1, 16);
+    add_instance<H, F, 256, 256, 16, 4, 4, 1>(v, 2, 4);
+    add_instance<A, F, 256, 256, 32, 8, 1, 1>(v, 1, 8);
+    add_instance<A, F, 256, 256, 32, 16, 1, 1>(v, 1, 16);
+    add_instance<A, F, 256, 256, 32, 4, 4, 1>(v, 2, 4);
+}
+
+}  // namespace kami

@@ -1,0 +1,61 @@
+// Registry of compiled block-GEMM instances.
+//
+// The reference fixes shape, partition and wave count with -D macros per executable
+// (scripts/fp64_block_square_GH200.sh:11-25, scripts/fp16_block_square_GH200.sh:11-25).
+// Here every (precision, M, N, K, partition, P) instance is compiled once into the
+// library and selected at run time from the descriptor.
+#pragma once
+
+#include <vector>
+
+#include "block_gemm_kernel.hpp"
+
+namespace kami {
+
+typedef hipError_t (*block_launch_fn)(const BlockArgs&, int64_t, hipStream_t);
+
+struct BlockInstance {
+    int prec;       // kami_precision
+    int m, n, k;
+    int part;       // kami_partition
+    int ranks;      // P as the reference counts it
+    int pm, pn, pk; // wave grid
+    int ks;         // K staged per LDS stage
+    int threads;
+    int lds;
+    int atom_k;     // K of one fragment step
+    block_launch_fn launch;
+};
+
+// Largest K stage (a divisor of BK, a multiple of PK * atom K step) whose LDS staging
+// fits `budget` bytes; whole-K staging uses one buffer, partial staging two.
+template <class AT, int BM, int BN, int BK, int PM, int PN, int PK>
+constexpr int choose_ks(int budget) {
+    constexpr int elt = (int)sizeof(typename AT::in_t);
+    int best = 0;
+    for (int ks = BK; ks >= PK * AT::KSTEP; ks /= 2) {
+        if (BK % ks != 0 || ks % (PK * AT::KSTEP) != 0) continue;
+        const int rowb = ks * elt + 16;
+        const int stage = ((PN == 1) ? 0 : BM * rowb) + BN * rowb;
+        const int nbuf = (ks == BK) ? 1 : 2;
+        best = ks;
+        if (nbuf * stage <= budget) return ks;
+    }
+    return best;
+}
+
+constexpr int kStageBudget = 80 * 1024;
+
+template <class AT, int PREC, int BM, int BN, int BK, int PM, int PN, int PK>
+void add_instance(std::vector<BlockInstance>& v, int part, int ranks) {
+    constexpr int KS = choose_ks<AT, BM, BN, BK, PM, PN, PK>(kStageBudget);
+    using G = BlockGeom<AT, BM, BN, BK, PM, PN, PK, KS>;
+    v.push_back(BlockInstance{PREC, BM, BN, BK, part, ranks, PM, PN, PK, KS, G::NT, G::LDS,
+                              AT::KSTEP, &launch_block_gemm<AT, G>});
+}
+
+void register_f64_instances(std::vector<BlockInstance>& v);
+void register_f16_instances(std::vector<BlockInstance>& v);
+void register_bf16_instances(std::vector<BlockInstance>& v);
+
+}  // namespace kami

@@ -1,0 +1,296 @@
+// C ABI: status handling, descriptor validation, instance selection, block-GEMM launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/kami.h"
+#include "block_gemm_registry.hpp"
+#include "kami_internal.hpp"
+
+namespace kami {
+
+thread_local int32_t g_last_hip = 0;
+
+kami_status hip_fail(hipError_t e) {
+    g_last_hip = (int32_t)e;
+    return KAMI_EHIP;
+}
+
+static std::once_flag g_reg_once;
+static std::vector<BlockInstance>* g_reg = nullptr;
+
+const std::vector<BlockInstance>& registry() {
+    std::call_once(g_reg_once, [] {
+        auto* v = new std::vector<BlockInstance>();
+        register_f64_instances(*v);
+        register_f16_instances(*v);
+        register_bf16_instances(*v);
+        g_reg = v;
+    });
+    return *g_reg;
+}
+
+int elt_in(int prec) {
+    switch (prec) {
+        case KAMI_PREC_F16_F32:
+        case KAMI_PREC_BF16_F32: return 2;
+        default: return 8;
+    }
+}
+int elt_out(int prec) {
+    switch (prec) {
+        case KAMI_PREC_F16_F32:
+        case KAMI_PREC_BF16_F32: return 4;
+        default: return 8;
+    }
+}
+
+// Per-device gfx950 check, cached.
+kami_status device_check() {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        g_last_hip = (int32_t)e;
+        return KAMI_ENODEV;
+    }
+    static std::atomic<int> cache[64];  // 0 unknown, 1 ok, 2 not gfx950
+    if (dev >= 0 && dev < 64) {
+        int c = cache[dev].load(std::memory_order_relaxed);
+        if (c == 1) return KAMI_OK;
+        if (c == 2) return KAMI_ENODEV;
+    }
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) {
+        g_last_hip = (int32_t)e;
+        return KAMI_ENODEV;
+    }
+    const bool ok = std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+    if (dev >= 0 && dev < 64) cache[dev].store(ok ? 1 : 2, std::memory_order_relaxed);
+    return ok ? KAMI_OK : KAMI_ENODEV;
+}
+
+// Default wave-grid edge when the caller leaves ranks = 0, per partition.  These are
+// the reference's sweep tables (scripts/fp64_block_square_GH200.sh:11-25,
+// scripts/fp16_block_square_GH200.sh:11-25) mapped to wave64: 1D keeps M/16 waves of
+// 16 rows for FP64; FP16 uses the fewest waves whose accumulators fit.
+static int default_ranks(int prec, int part, int m, int n, int k) {
+    (void)k;
+    const bool f64 = (prec == KAMI_PREC_F64 || prec == KAMI_PREC_F64_SPLIT_F16 ||
+                      prec == KAMI_PREC_F64_SPLIT_BF16);
+    const int mn = std::max(m, n);
+    if (part == KAMI_PART_1D) {
+        if (f64) return std::max(1, m / 16);
+        if (mn <= 64) return 1;
+        if (mn <= 128) return 4;
+        return 8;
+    }
+    if (part == KAMI_PART_2D) {
+        if (mn <= 32) return 1;
+        if (mn <= 128) return 2;
+        return 4;
+    }
+    // 3D
+    if (mn <= 16) return 1;
+    return 2;
+}
+
+// Partition chosen for KAMI_PART_AUTO before any tuning data exists: 1D, the
+// reference's best FP64 and FP16 partition at these sizes
+// (logs/block_gemm/square/H200/fp64_block_square_1d_H200.csv:4,
+//  logs/block_gemm/square/H200/fp16_block_square_1d_H200.csv:4,8).
+static std::mutex g_tune_mu;
+struct TuneKey {
+    int prec, m, n, k;
+    bool operator==(const TuneKey& o) const {
+        return prec == o.prec && m == o.m && n == o.n && k == o.k;
+    }
+};
+struct TuneVal {
+    int part, ranks;
+};
+static std::vector<std::pair<TuneKey, TuneVal>> g_tuned;
+
+void set_tuned_choice(int prec, int m, int n, int k, int part, int ranks) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (auto& kv : g_tuned)
+        if (kv.first == TuneKey{prec, m, n, k}) {
+            kv.second = TuneVal{part, ranks};
+            return;
+        }
+    g_tuned.push_back({TuneKey{prec, m, n, k}, TuneVal{part, ranks}});
+}
+
+static bool get_tuned_choice(int prec, int m, int n, int k, TuneVal* out) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (auto& kv : g_tuned)
+        if (kv.first == TuneKey{prec, m, n, k}) {
+            *out = kv.second;
+            return true;
+        }
+    return false;
+}
+
+kami_status validate_desc(const kami_gemm_desc* d) {
+    if (!d) return KAMI_EINVAL;
+    if (d->m <= 0 || d->n <= 0 || d->k <= 0) return KAMI_EINVAL;
+    if (d->precision < KAMI_PREC_F64 || d->precision > KAMI_PREC_F64_SPLIT_BF16) return KAMI_EINVAL;
+    if (d->partition < KAMI_PART_AUTO || d->partition > KAMI_PART_3D) return KAMI_EINVAL;
+    if (d->ranks < 0 || d->iters < 1) return KAMI_EINVAL;
+    if (d->b_layout != KAMI_B_NK && d->b_layout != KAMI_B_KN) return KAMI_EINVAL;
+    if (d->lda < 0 || d->ldb < 0 || d->ldc < 0 || d->slices < 0) return KAMI_EINVAL;
+    const int64_t lda = d->lda ? d->lda : d->k;
+    const int64_t ldb = d->ldb ? d->ldb : (d->b_layout == KAMI_B_NK ? d->k : d->n);
+    const int64_t ldc = d->ldc ? d->ldc : d->n;
+    if (lda < d->k) return KAMI_EINVAL;
+    if (ldb < (d->b_layout == KAMI_B_NK ? d->k : d->n)) return KAMI_EINVAL;
+    if (ldc < d->n) return KAMI_EINVAL;
+    const int e = elt_in(d->precision);
+    // operands are staged with 16-byte loads
+    if ((lda * e) % 16 != 0 || (ldb * e) % 16 != 0) return KAMI_EINVAL;
+    return KAMI_OK;
+}
+
+const BlockInstance* resolve_block(const kami_gemm_desc* d, kami_status* st) {
+    int part = d->partition, ranks = d->ranks;
+    if (part == KAMI_PART_AUTO) {
+        TuneVal tv;
+        if (get_tuned_choice(d->precision, d->m, d->n, d->k, &tv)) {
+            part = tv.part;
+            ranks = tv.ranks;
+        } else {
+            part = KAMI_PART_1D;
+            ranks = 0;
+        }
+    }
+    if (ranks == 0) ranks = default_ranks(d->precision, part, d->m, d->n, d->k);
+    for (const auto& in : registry()) {
+        if (in.prec == d->precision && in.m == d->m && in.n == d->n && in.k == d->k &&
+            in.part == part && in.ranks == ranks) {
+            *st = KAMI_OK;
+            return &in;
+        }
+    }
+    // AUTO falls back to any compiled instance of the shape (first registered)
+    if (d->partition == KAMI_PART_AUTO) {
+        for (const auto& in : registry())
+            if (in.prec == d->precision && in.m == d->m && in.n == d->n && in.k == d->k) {
+                *st = KAMI_OK;
+                return &in;
+            }
+    }
+    *st = KAMI_ENOTSUP;
+    return nullptr;
+}
+
+}  // namespace kami
+
+using namespace kami;
+
+extern "C" {
+
+const char* kami_status_string(kami_status s) {
+    switch (s) {
+        case KAMI_OK: return "KAMI_OK: success";
+        case KAMI_EINVAL: return "KAMI_EINVAL: invalid argument";
+        case KAMI_ENOTSUP: return "KAMI_ENOTSUP: no compiled instance for this request";
+        case KAMI_ENOMEM: return "KAMI_ENOMEM: allocation failed";
+        case KAMI_EHIP: return "KAMI_EHIP: HIP runtime error";
+        case KAMI_ENOSPC: return "KAMI_ENOSPC: workspace too small";
+        case KAMI_ENODEV: return "KAMI_ENODEV: no gfx950 device";
+    }
+    return "KAMI: unknown status";
+}
+
+int32_t kami_last_hip_error(void) { return g_last_hip; }
+
+const char* kami_version(void) { return "kami-mi355x 0.1 (gfx950)"; }
+
+kami_status kami_device_check(void) { return device_check(); }
+
+kami_status kami_gemm_desc_init(kami_gemm_desc* d, int32_t m, int32_t n, int32_t k,
+                                int32_t precision) {
+    if (!d) return KAMI_EINVAL;
+    std::memset(d, 0, sizeof(*d));
+    d->m = m;
+    d->n = n;
+    d->k = k;
+    d->precision = precision;
+    d->partition = KAMI_PART_AUTO;
+    d->ranks = 0;
+    d->b_layout = KAMI_B_NK;
+    d->iters = 1;
+    d->alpha = 1.0;
+    d->beta = 0.0;
+    return KAMI_OK;
+}
+
+kami_status kami_block_gemm_query(const kami_gemm_desc* d, kami_gemm_config* out) {
+    kami_status st = validate_desc(d);
+    if (st != KAMI_OK) return st;
+    if (!out) return KAMI_EINVAL;
+    if (is_split(d->precision)) return split_query(d, out);
+    const BlockInstance* in = resolve_block(d, &st);
+    if (!in) return st;
+    out->partition = in->part;
+    out->ranks = in->ranks;
+    out->waves_m = in->pm;
+    out->waves_n = in->pn;
+    out->waves_k = in->pk;
+    out->threads = in->threads;
+    out->k_stage = in->ks;
+    out->lds_bytes = in->lds;
+    out->slices = 0;
+    out->instance = (int32_t)(in - registry().data());
+    return KAMI_OK;
+}
+
+kami_status kami_block_gemm(const kami_gemm_desc* d, const void* A, const void* B, void* C,
+                            int64_t count, int64_t stride_a, int64_t stride_b, int64_t stride_c,
+                            void* stream) {
+    kami_status st = validate_desc(d);
+    if (st != KAMI_OK) return st;
+    if (count < 0 || count > 0x7fffffffLL) return KAMI_EINVAL;
+    if (stride_a < 0 || stride_b < 0 || stride_c < 0) return KAMI_EINVAL;
+    if (count == 0) return KAMI_OK;
+    if (!A || !B || !C) return KAMI_EINVAL;
+    const int ei = elt_in(d->precision), eo = elt_out(d->precision);
+    if (((uintptr_t)A % 16) || ((uintptr_t)B % 16) || ((uintptr_t)C % eo)) return KAMI_EINVAL;
+    BlockArgs args;
+    args.A = A;
+    args.B = B;
+    args.C = C;
+    args.lda = d->lda ? d->lda : d->k;
+    args.ldb = d->ldb ? d->ldb : (d->b_layout == KAMI_B_NK ? d->k : d->n);
+    args.ldc = d->ldc ? d->ldc : d->n;
+    const int64_t a_span = (int64_t)(d->m - 1) * args.lda + d->k;
+    const int64_t b_span = d->b_layout == KAMI_B_NK ? (int64_t)(d->n - 1) * args.ldb + d->k
+                                                    : (int64_t)(d->k - 1) * args.ldb + d->n;
+    const int64_t c_span = (int64_t)(d->m - 1) * args.ldc + d->n;
+    args.sa = stride_a ? stride_a : (d->lda ? a_span : (int64_t)d->m * d->k);
+    args.sb = stride_b ? stride_b : (d->ldb ? b_span : (int64_t)d->n * d->k);
+    args.sc = stride_c ? stride_c : (d->ldc ? c_span : (int64_t)d->m * d->n);
+    if (count > 1 && ((args.sa * ei) % 16 || (args.sb * ei) % 16)) return KAMI_EINVAL;
+    args.alpha = d->alpha;
+    args.beta = d->beta;
+    args.iters = d->iters;
+    args.b_kn = d->b_layout == KAMI_B_KN;
+    if (is_split(d->precision)) {
+        st = device_check();
+        if (st != KAMI_OK) return st;
+        return split_launch(d, args, count, (hipStream_t)stream);
+    }
+    const BlockInstance* in = resolve_block(d, &st);
+    if (!in) return st;
+    st = device_check();
+    if (st != KAMI_OK) return st;
+    hipError_t e = in->launch(args, count, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e);
+    return KAMI_OK;
+}
+
+}  // extern "C"

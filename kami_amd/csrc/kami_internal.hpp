@@ -1,0 +1,31 @@
+// Internal declarations shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/kami.h"
+#include "block_gemm_kernel.hpp"
+#include "block_gemm_registry.hpp"
+
+namespace kami {
+
+extern thread_local int32_t g_last_hip;
+kami_status hip_fail(hipError_t e);
+kami_status device_check();
+const std::vector<BlockInstance>& registry();
+kami_status validate_desc(const kami_gemm_desc* d);
+const BlockInstance* resolve_block(const kami_gemm_desc* d, kami_status* st);
+void set_tuned_choice(int prec, int m, int n, int k, int part, int ranks);
+int elt_in(int prec);
+int elt_out(int prec);
+
+inline bool is_split(int prec) {
+    return prec == KAMI_PREC_F64_SPLIT_F16 || prec == KAMI_PREC_F64_SPLIT_BF16;
+}
+kami_status split_query(const kami_gemm_desc* d, kami_gemm_config* out);
+kami_status split_launch(const kami_gemm_desc* d, const BlockArgs& args, int64_t count,
+                         hipStream_t stream);
+
+}  // namespace kami

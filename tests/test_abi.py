@@ -1,0 +1,100 @@
+"""C-ABI checks that run without a GPU: exports, validation, instance selection."""
+import ctypes
+
+import pytest
+
+import kami_amd as K
+from kami_amd import _abi
+
+
+def test_library_exports_every_header_function():
+    names = _abi.header_functions()
+    assert len(names) >= 15
+    lib = K.lib()
+    for n in names:
+        assert hasattr(lib, n), f"libkami.so does not export {n}"
+
+
+def test_status_strings_and_version():
+    lib = K.lib()
+    for s in range(7):
+        assert lib.kami_status_string(s).decode().startswith("KAMI_")
+    assert "gfx950" in K.version()
+
+
+def test_struct_sizes_match_header_layout():
+    assert ctypes.sizeof(_abi.GemmDesc) == 80
+    assert ctypes.sizeof(_abi.GemmConfig) == 40
+    assert ctypes.sizeof(_abi.Problem) == 64
+    assert ctypes.sizeof(_abi.BatchedOpts) == 32
+    assert ctypes.sizeof(_abi.Bsr) == 40
+
+
+def _st(d, A=1 << 20, B=1 << 20, C=1 << 20, count=1):
+    return K.lib().kami_block_gemm(ctypes.byref(d), ctypes.c_void_p(A), ctypes.c_void_p(B),
+                                   ctypes.c_void_p(C), count, 0, 0, 0, None)
+
+
+@pytest.mark.parametrize("field,value", [("m", 0), ("n", -1), ("k", 0), ("precision", 9),
+                                         ("partition", 7), ("ranks", -1), ("iters", 0),
+                                         ("b_layout", 3), ("lda", 8), ("ldb", -1)])
+def test_invalid_descriptor_is_einval(field, value):
+    d = K.make_desc(64, 64, 64, K.PREC_F64)
+    setattr(d, field, value)
+    assert _st(d) == _abi.EINVAL
+
+
+def test_invalid_pointers_and_counts():
+    d = K.make_desc(64, 64, 64, K.PREC_F64)
+    assert K.lib().kami_block_gemm(ctypes.byref(d), None, ctypes.c_void_p(16), ctypes.c_void_p(16),
+                                   1, 0, 0, 0, None) == _abi.EINVAL
+    assert _st(d, A=(1 << 20) + 8) == _abi.EINVAL          # A not 16-byte aligned
+    assert _st(d, count=-1) == _abi.EINVAL
+    assert _st(d, count=0) == _abi.OK                       # empty batch is a no-op
+    d.lda = 65                                              # 65 * 8 bytes: not a 16B multiple
+    assert _st(d) == _abi.EINVAL
+    d.lda = 68
+    d.ldc = 32                                              # ldc < n
+    assert _st(d) == _abi.EINVAL
+
+
+def test_unsupported_shape_is_enotsup():
+    d = K.make_desc(48, 48, 48, K.PREC_F64)
+    cfg = _abi.GemmConfig()
+    assert K.lib().kami_block_gemm_query(ctypes.byref(d), ctypes.byref(cfg)) == _abi.ENOTSUP
+    d = K.make_desc(256, 256, 256, K.PREC_F16_F32, partition="3d")
+    assert K.lib().kami_block_gemm_query(ctypes.byref(d), ctypes.byref(cfg)) == _abi.ENOTSUP
+
+
+@pytest.mark.parametrize("prec,s,part,waves", [
+    (K.PREC_F64, 64, "1d", (4, 1, 1)), (K.PREC_F64, 64, "2d", (2, 2, 1)),
+    (K.PREC_F64, 64, "3d", (2, 2, 2)), (K.PREC_F64, 128, "1d", (8, 1, 1)),
+    (K.PREC_F16_F32, 128, "1d", (4, 1, 1)), (K.PREC_F16_F32, 256, "2d", (4, 4, 1)),
+    (K.PREC_BF16_F32, 64, "3d", (2, 2, 2)),
+])
+def test_query_default_ranks_follow_reference_tables(prec, s, part, waves):
+    cfg = K.query(s, s, s, prec, partition=part)
+    assert (cfg["waves_m"], cfg["waves_n"], cfg["waves_k"]) == waves
+    assert cfg["threads"] == 64 * waves[0] * waves[1] * waves[2]
+    assert 0 < cfg["lds_bytes"] <= 160 * 1024
+
+
+def test_every_instance_fits_lds_and_workgroup_limits():
+    for prec, sizes in [(K.PREC_F64, [16, 32, 64, 128]),
+                        (K.PREC_F16_F32, [32, 64, 128, 256]),
+                        (K.PREC_BF16_F32, [32, 64, 128, 256])]:
+        for s in sizes:
+            for part in ("1d", "2d", "3d"):
+                try:
+                    cfg = K.query(s, s, s, prec, partition=part)
+                except K.KamiError as e:
+                    assert e.status == _abi.ENOTSUP
+                    continue
+                assert cfg["threads"] <= 1024 and cfg["lds_bytes"] <= 160 * 1024
+
+
+def test_device_check_without_gpu_reports_nodev():
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    assert K.lib().kami_device_check() == _abi.ENODEV
