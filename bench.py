@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X block-GEMM path (one JSON line on stdout).
+
+Default workload: FP64 block GEMM C = A B^T at M = N = K = 128 over 4096 independent
+problems (one per workgroup), entries sign-random with log10|x| ~ U[-4, 4].  It is timed
+two ways on the same inputs: native FP64 MFMA and FP64 through FP16 slices on MFMA
+(split precision).  `value` is the TFLOPS (2MNK per problem, FP64-equivalent) of the
+path named by --path (default: the faster of the two that meets the 1e-13 error bound).
+
+Inputs are resident in HBM before timing.  W untimed warm-up steps, then K timed steps
+bracketed by device synchronisation; one step is one library call over the whole batch.
+The roofline object describes the dominant kernel of the reported path; the CPU
+baseline times the oracle's OpenMP FP64 triple loop (the reference's CPU check loop)
+on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"f64_mfma_tflops": 78.6, "f16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0}
+SEED = 0x4B414D49
+
+METRIC = "Batched block-GEMM TFLOPS & % MFMA peak, FP64 native vs split-FP16; max rel err"
+
+
+def log_uniform_signed(rng, shape, lo=-4.0, hi=4.0):
+    mag = 10.0 ** rng.uniform(lo, hi, size=shape)
+    sign = np.where(rng.random(shape) < 0.5, -1.0, 1.0)
+    return sign * mag
+
+
+def time_calls(fn, steps, warmup, stream):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        fn()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps  # ms per step
+
+
+def cpu_baseline_f64(m, n, k, count, budget_s=20.0):
+    """The oracle's OpenMP FP64 triple loop on a bounded sample of the same workload."""
+    import oracle
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    rng = np.random.default_rng(SEED + 99)
+    probe = 32
+    A = log_uniform_signed(rng, (probe, m, k))
+    B = log_uniform_signed(rng, (probe, n, k))
+    t0 = time.perf_counter()
+    oracle.gemm_f64(A, B, nthreads=cores)
+    dt = time.perf_counter() - t0
+    sample = int(min(count, max(probe, probe * budget_s / max(dt, 1e-6) / 2)))
+    A = log_uniform_signed(rng, (sample, m, k))
+    B = log_uniform_signed(rng, (sample, n, k))
+    t0 = time.perf_counter()
+    oracle.gemm_f64(A, B, nthreads=cores)
+    dt = time.perf_counter() - t0
+    tflops = 2.0 * m * n * k * sample / dt / 1e12
+    return {"value": tflops, "unit": "TFLOPS", "cores": cores, "kind": "port",
+            "sample": f"{sample} of {count} problems {m}x{n}x{k} FP64, oracle triple loop "
+                      f"(OpenMP over problems), {dt:.2f} s"}
+
+
+def rel_errors(C, A, B, sample):
+    """Max errors of C against the compensated (Dot2) oracle on the first `sample`
+    problems: componentwise |dC| / (|A||B|) and normwise max|dC| / max|C*| per problem."""
+    import oracle
+    exact, absC = oracle.gemm_dot2(A[:sample], B[:sample])
+    d = np.abs(C[:sample] - exact)
+    comp = float(np.max(d / np.where(absC > 0, absC, 1.0)))
+    norm = float(np.max(np.max(d, axis=(1, 2)) / np.maximum(np.max(np.abs(exact), axis=(1, 2)), 1e-300)))
+    return {"componentwise": comp, "normwise": norm}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--path", choices=["auto", "native", "split"], default="auto")
+    ap.add_argument("--m", type=int, default=128)
+    ap.add_argument("--problems", type=int, default=4096)
+    ap.add_argument("--err-sample", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the batched path shards by problem across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import kami_amd as K
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    m = n = k = args.m
+    count = args.problems
+    rng = np.random.default_rng(SEED + 2)
+    A = log_uniform_signed(rng, (count, m, k))
+    B = log_uniform_signed(rng, (count, n, k))
+    Ad = torch.from_numpy(A).to(dev)
+    Bd = torch.from_numpy(B).to(dev)
+    flops = 2.0 * m * n * k * count
+    bytes_alg = 8.0 * (m * k + n * k + m * n) * count
+
+    results = {}
+    outs = {}
+    paths = [("native", K.PREC_F64), ("split", K.PREC_F64_SPLIT_F16)]
+    for name, prec in paths:
+        try:
+            cfg = K.query(m, n, k, prec)
+        except K.KamiError as e:
+            results[name] = {"available": False, "reason": str(e)}
+            continue
+        Cd = torch.empty((count, m, n), dtype=torch.float64, device=dev)
+        fn = lambda: K.block_gemm(Ad, Bd, Cd, precision=prec, stream=stream)  # noqa: E731
+        ms = time_calls(fn, args.steps, args.warmup, stream)
+        C = Cd.cpu().numpy()
+        outs[name] = C
+        err = rel_errors(C, A, B, min(args.err_sample, count))
+        tflops = flops / (ms * 1e-3) / 1e12
+        results[name] = {"available": True, "ms": ms, "tflops": tflops,
+                         "frac_f64_peak": tflops / PEAK["f64_mfma_tflops"],
+                         "hbm_gbs": bytes_alg / (ms * 1e-3) / 1e9, "max_rel_err": err,
+                         "config": cfg}
+        print(f"[bench] {name}: {ms:.4f} ms/step  {tflops:.2f} TFLOPS  err={err}",
+              file=sys.stderr, flush=True)
+
+    ok = [p for p in ("split", "native") if results.get(p, {}).get("available")
+          and (p == "native" or results[p]["max_rel_err"]["normwise"] <= 1e-13)]
+    if args.path == "auto":
+        chosen = max(ok, key=lambda p: results[p]["tflops"])
+    else:
+        chosen = args.path
+        if not results.get(chosen, {}).get("available"):
+            print(f"bench.py: path {chosen} unavailable", file=sys.stderr)
+            sys.exit(1)
+    r = results[chosen]
+    ms = r["ms"]
+    if chosen == "native":
+        roof = {"bound": "mfma", "achieved": r["tflops"], "peak": PEAK["f64_mfma_tflops"],
+                "unit": "TFLOP/s", "frac": r["tflops"] / PEAK["f64_mfma_tflops"], "traffic": None}
+    else:
+        gbs = r["hbm_gbs"]
+        roof = {"bound": "hbm", "achieved": gbs, "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+                "frac": gbs / PEAK["hbm_gbs"], "traffic": None}
+    cpu = None if args.no_cpu else cpu_baseline_f64(m, n, k, count)
+    line = {
+        "metric": METRIC,
+        "value": r["tflops"],
+        "unit": "TFLOPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if chosen == "native" else "f16-slices/f32-acc -> f64",
+        "data": "synthetic",
+        "config": {"workload": f"FP64 block GEMM {m}x{n}x{k}, {count} problems, log-uniform "
+                               f"|x| in [1e-4, 1e4] with random signs; path={chosen}",
+                   "m": m, "n": n, "k": k, "problems": count,
+                   "partition": r["config"]["partition"], "ranks": r["config"]["ranks"]},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "paths": {p: ({kk: vv for kk, vv in v.items() if kk != "config"} if v.get("available")
+                      else v) for p, v in results.items()},
+        "max_rel_err": r["max_rel_err"],
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

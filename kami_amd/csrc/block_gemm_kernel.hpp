@@ -230,12 +230,28 @@ block_gemm_kernel(BlockArgs args) {
                         load_raw<FB>(base + (pm * G::TM + im * 16 + (lane & 15)) * G::ROWB + kk));
                 }
             }
+            if constexpr (AT::NSUB == 1) {
 #pragma unroll
-            for (int jn = 0; jn < G::JN; ++jn) {
-                const frag_t bf = bitcast<frag_t>(
-                    load_raw<FB>(bb + (pn * G::TN + jn * 16 + (lane & 15)) * G::ROWB + kk));
+                for (int jn = 0; jn < G::JN; ++jn) {
+                    const frag_t bf = bitcast<frag_t>(
+                        load_raw<FB>(bb + (pn * G::TN + jn * 16 + (lane & 15)) * G::ROWB + kk));
 #pragma unroll
-                for (int im = 0; im < G::IM; ++im) AT::mma(acc[im][jn], af[im], bf);
+                    for (int im = 0; im < G::IM; ++im) AT::mma(acc[im][jn], af[im], bf);
+                }
+            } else {
+                // several instructions per fragment: issue sub-step e for every atom
+                // before sub-step e + 1 so consecutive MFMAs never share an accumulator
+                frag_t bf[G::JN];
+#pragma unroll
+                for (int jn = 0; jn < G::JN; ++jn)
+                    bf[jn] = bitcast<frag_t>(
+                        load_raw<FB>(bb + (pn * G::TN + jn * 16 + (lane & 15)) * G::ROWB + kk));
+#pragma unroll
+                for (int e = 0; e < AT::NSUB; ++e)
+#pragma unroll
+                    for (int jn = 0; jn < G::JN; ++jn)
+#pragma unroll
+                        for (int im = 0; im < G::IM; ++im) AT::mma_sub(acc[im][jn], af[im], bf[jn], e);
             }
         }
     };

@@ -47,9 +47,13 @@ struct AtomF64 {
     static constexpr int KV = 2;
     static constexpr int KSTEP = 8;
     static constexpr int FRAG_BYTES = 16;
+    static constexpr int NSUB = 2;  // instructions per fragment
+    __device__ static inline void mma_sub(accv_t& c, const frag_t& a, const frag_t& b, int e) {
+        c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[e], b[e], c, 0, 0, 0);
+    }
     __device__ static inline void mma(accv_t& c, const frag_t& a, const frag_t& b) {
-        c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[0], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b[1], c, 0, 0, 0);
+        mma_sub(c, a, b, 0);
+        mma_sub(c, a, b, 1);
     }
     __device__ static inline int out_row(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
@@ -64,6 +68,10 @@ struct AtomF16x32 {
     static constexpr int KV = 8;
     static constexpr int KSTEP = 32;
     static constexpr int FRAG_BYTES = 16;
+    static constexpr int NSUB = 1;
+    __device__ static inline void mma_sub(accv_t& c, const frag_t& a, const frag_t& b, int) {
+        mma(c, a, b);
+    }
     __device__ static inline void mma(accv_t& c, const frag_t& a, const frag_t& b) {
         c = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
     }
@@ -80,6 +88,10 @@ struct AtomF16x16 {
     static constexpr int KV = 4;
     static constexpr int KSTEP = 16;
     static constexpr int FRAG_BYTES = 8;
+    static constexpr int NSUB = 1;
+    __device__ static inline void mma_sub(accv_t& c, const frag_t& a, const frag_t& b, int) {
+        mma(c, a, b);
+    }
     __device__ static inline void mma(accv_t& c, const frag_t& a, const frag_t& b) {
         c = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
     }
@@ -96,6 +108,10 @@ struct AtomBF16x32 {
     static constexpr int KV = 8;
     static constexpr int KSTEP = 32;
     static constexpr int FRAG_BYTES = 16;
+    static constexpr int NSUB = 1;
+    __device__ static inline void mma_sub(accv_t& c, const frag_t& a, const frag_t& b, int) {
+        mma(c, a, b);
+    }
     __device__ static inline void mma(accv_t& c, const frag_t& a, const frag_t& b) {
         c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
     }
@@ -112,6 +128,10 @@ struct AtomBF16x16 {
     static constexpr int KV = 4;
     static constexpr int KSTEP = 16;
     static constexpr int FRAG_BYTES = 8;
+    static constexpr int NSUB = 1;
+    __device__ static inline void mma_sub(accv_t& c, const frag_t& a, const frag_t& b, int) {
+        mma(c, a, b);
+    }
     __device__ static inline void mma(accv_t& c, const frag_t& a, const frag_t& b) {
         c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
     }

@@ -1,21 +1,352 @@
-// Split-precision FP64 block GEMM (placeholder until the slicing kernels land).
+// FP64 block GEMM through FP16 / BF16 slices on MFMA (split precision).
+//
+// C = A * B^T in FP64 computed as an error-free sum of low-precision slice products
+// (the Ozaki scheme: Ozaki, Ogita, Oishi, Rump, "Error-free transformations of matrix
+// multiplication by using fast routines of matrix multiplication and its applications",
+// Numer. Algorithms 59 (2012); tensor-core form: Mukunoki, Ozaki, Ogita, Imamura,
+// ICCS 2020).  No counterpart exists in the reference; it is the split-precision FP64
+// path the block_gemm interface offers beside native FP64 MFMA.
+//
+// Per problem (one workgroup):
+//   1. exponents: e_i = exponent of max_k |A(i,k)| for every row of A and f_j of every
+//      row of B (B is stored N x K), so |A(i,k)| < 2^e_i;
+//   2. slicing, per 32-wide K chunk: A(i,k) 2^(BETA - e_i) = sum_s a_s 2^(-BETA (s-1))
+//      with integer digits a_s chosen by round-to-nearest (|a_1| <= 2^BETA,
+//      |a_s| <= 2^(BETA-1) after that), exactly representable in FP16 (BETA = 9) or
+//      BF16 (BETA = 8); the digits go to LDS, one plane per slice;
+//   3. products: for every level g = s + t <= G the FP32 MFMA sum
+//      T_g = sum_{s+t=g} a_s b_t^T over the 32-wide chunk is an integer of magnitude
+//      <= 2^24 (checked at compile time for the levels that matter), hence exact in
+//      any summation order; it is drained into an FP64 accumulator with weight
+//      2^(-BETA g) after each chunk;
+//   4. C(i,j) = alpha 2^(e_i + f_j) acc(i,j) + beta C(i,j).
+// The error relative to 2^(e_i + f_j) is about K 2^(-BETA S - 1) from the truncated
+// digits plus the dropped levels g > G; finite inputs are required.
+#include <cmath>
+
 #include "kami_internal.hpp"
 
 namespace kami {
 
-kami_status split_query(const kami_gemm_desc* d, kami_gemm_config* out) {
-    (void)d;
-    (void)out;
-    return KAMI_ENOTSUP;
+// Largest |digit| sum bound of level g for BETA-bit signed digits over KC terms, in
+// units of 2^0, compared against 2^24 (the FP32 integer range).
+constexpr double level_bound(int beta, int S, int g, int kc) {
+    double tot = 0;
+    for (int s = 1; s <= S; ++s) {
+        const int t = g - s;
+        if (t < 1 || t > S) continue;
+        const double bs = (s == 1) ? (double)(1 << beta) : (double)(1 << (beta - 1));
+        const double bt = (t == 1) ? (double)(1 << beta) : (double)(1 << (beta - 1));
+        tot += kc * bs * bt;
+    }
+    return tot;
 }
 
-kami_status split_launch(const kami_gemm_desc* d, const BlockArgs& args, int64_t count,
+// Every level whose FP32 rounding could matter is exact: a level that may exceed 2^24
+// is allowed only when its worst rounding, 2^-24 * bound * 2^(-beta g) in units of
+// 2^(e_i + f_j), stays below 2^-50.
+constexpr bool levels_exact(int beta, int S, int G, int kc) {
+    for (int g = 2; g <= G; ++g) {
+        const double b = level_bound(beta, S, g, kc);
+        if (b <= 16777216.0) continue;
+        double rel = b;
+        for (int q = 0; q < 24 + beta * g; ++q) rel *= 0.5;
+        double lim = 1.0;
+        for (int q = 0; q < 50; ++q) lim *= 0.5;
+        if (rel > lim) return false;
+    }
+    return true;
+}
+
+template <class SAT, int BM_, int BN_, int BK_, int PM_, int PN_, int BETA_, int S_, int G_>
+struct SplitGeom {
+    static constexpr int BM = BM_, BN = BN_, BK = BK_, PM = PM_, PN = PN_;
+    static constexpr int BETA = BETA_, S = S_, G = G_;
+    static constexpr int WAVES = PM * PN, NT = WAVES * kWave;
+    static constexpr int TM = BM / PM, TN = BN / PN, IM = TM / 16, JN = TN / 16;
+    static constexpr int JH = JN < 2 ? JN : 2;  // column atoms per register group
+    static constexpr int KC = 32, NCH = BK / KC;
+    static constexpr int ROWB = KC * 2 + 16;  // 32 digits + 16 B pad: conflict-free b128 reads
+    static constexpr int SA_BYTES = BM * ROWB, SB_BYTES = BN * ROWB;
+    static constexpr int PLANE = SA_BYTES + SB_BYTES;
+    static constexpr int EXP_OFF = S * PLANE;
+    static constexpr int LDS = EXP_OFF + (BM + BN) * 4;
+    static constexpr int PA = BM * 16 / NT, PB = BN * 16 / NT;  // 16-byte pieces per thread
+    static_assert(BM % PM == 0 && BN % PN == 0 && TM % 16 == 0 && TN % 16 == 0, "tiling");
+    static_assert(BK % KC == 0, "K must be a multiple of 32");
+    static_assert((BM * 16) % NT == 0 && (BN * 16) % NT == 0, "piece distribution");
+    static_assert(LDS <= 160 * 1024, "LDS budget");
+    static_assert(levels_exact(BETA, S, G, KC), "slice digits would not sum exactly in FP32");
+};
+
+struct SplitArgs {
+    const double* A;
+    const double* B;
+    double* C;
+    int64_t sa, sb, sc, lda, ldb, ldc;
+    double alpha, beta;
+};
+
+__device__ inline int exp_of(double x) {
+    // exponent e with |x| in [2^(e-1), 2^e); zero maps to a sentinel below every
+    // finite exponent so it never raises a row maximum
+    return x == 0.0 ? -4096 : __builtin_amdgcn_frexp_exp(x);
+}
+
+template <class SAT>
+__device__ inline uint32_t pack2(double a, double b);
+
+template <>
+__device__ inline uint32_t pack2<AtomF16x32>(double a, double b) {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    h2 v = {(_Float16)(float)a, (_Float16)(float)b};  // small integers: exact
+    return __builtin_bit_cast(uint32_t, v);
+}
+
+template <>
+__device__ inline uint32_t pack2<AtomBF16x32>(double a, double b) {
+    // |digit| <= 256: exact in float and in bfloat16 (upper half of the float)
+    const uint32_t ua = __builtin_bit_cast(uint32_t, (float)a);
+    const uint32_t ub = __builtin_bit_cast(uint32_t, (float)b);
+    return (ua >> 16) | (ub & 0xffff0000u);
+}
+
+template <class SAT, class G>
+__global__ void __launch_bounds__(G::NT)
+split_gemm_kernel(SplitArgs args) {
+    using frag_t = typename SAT::frag_t;
+    using accv_t = typename SAT::accv_t;
+    __shared__ __attribute__((aligned(16))) unsigned char smem[G::LDS];
+    int* const expo = reinterpret_cast<int*>(smem + G::EXP_OFF);  // [BM] then [BN]
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int pn = wave % G::PN, pm = wave / G::PN;
+    const int64_t prob = blockIdx.x;
+    const double* __restrict__ A = args.A + prob * args.sa;
+    const double* __restrict__ B = args.B + prob * args.sb;
+    double* __restrict__ C = args.C + prob * args.sc;
+    const int64_t lda = args.lda, ldb = args.ldb, ldc = args.ldc;
+
+    // ---- 1. row exponents of A and B (one wave per row, lanes across K) ----
+    for (int r = wave; r < G::BM + G::BN; r += G::WAVES) {
+        const double* row = r < G::BM ? A + r * lda : B + (r - G::BM) * ldb;
+        int e = -4096;
+        for (int kp = lane; kp < G::BK / 2; kp += 64) {
+            const d2_t x = bitcast<d2_t>(load_raw<16>(row + 2 * kp));
+            e = max(e, max(exp_of(x[0]), exp_of(x[1])));
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) e = max(e, __shfl_xor(e, off));
+        if (lane == 0) expo[r] = (e == -4096) ? 0 : e;
+    }
+
+    double acc[G::IM][G::JN][4];
+#pragma unroll
+    for (int i = 0; i < G::IM; ++i)
+#pragma unroll
+        for (int j = 0; j < G::JN; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[i][j][v] = 0.0;
+
+    // ---- chunk pieces: 16 bytes = 2 doubles; 16 consecutive threads cover one row ----
+    u4_t xa[G::PA], xb[G::PB];
+    auto load_chunk = [&](int c) {
+#pragma unroll
+        for (int i = 0; i < G::PA; ++i) {
+            const int p = tid + i * G::NT, row = p >> 4, kp = p & 15;
+            xa[i] = load_raw<16>(A + row * lda + c * G::KC + 2 * kp);
+        }
+#pragma unroll
+        for (int i = 0; i < G::PB; ++i) {
+            const int p = tid + i * G::NT, row = p >> 4, kp = p & 15;
+            xb[i] = load_raw<16>(B + row * ldb + c * G::KC + 2 * kp);
+        }
+    };
+    auto slice_piece = [&](const u4_t& raw, int row, int kp, int e, unsigned char* plane0) {
+        const d2_t x = bitcast<d2_t>(raw);
+        double r0 = __builtin_amdgcn_ldexp(x[0], G::BETA - e);
+        double r1 = __builtin_amdgcn_ldexp(x[1], G::BETA - e);
+#pragma unroll
+        for (int s = 0; s < G::S; ++s) {
+            const double q0 = __builtin_rint(r0), q1 = __builtin_rint(r1);
+            *reinterpret_cast<uint32_t*>(plane0 + s * G::PLANE + row * G::ROWB + kp * 4) =
+                pack2<SAT>(q0, q1);
+            if (s + 1 < G::S) {
+                r0 = __builtin_amdgcn_ldexp(r0 - q0, G::BETA);
+                r1 = __builtin_amdgcn_ldexp(r1 - q1, G::BETA);
+            }
+        }
+    };
+
+    __syncthreads();  // exponents visible
+
+    for (int c = 0; c < G::NCH; ++c) {
+        load_chunk(c);
+        // ---- 2. slice the chunk into digit planes ----
+#pragma unroll
+        for (int i = 0; i < G::PA; ++i) {
+            const int p = tid + i * G::NT, row = p >> 4, kp = p & 15;
+            slice_piece(xa[i], row, kp, expo[row], smem);
+        }
+#pragma unroll
+        for (int i = 0; i < G::PB; ++i) {
+            const int p = tid + i * G::NT, row = p >> 4, kp = p & 15;
+            slice_piece(xb[i], row, kp, expo[G::BM + row], smem + G::SA_BYTES);
+        }
+        __syncthreads();
+
+        // ---- 3. level products on MFMA, drained to FP64 ----
+        frag_t af[G::S][G::IM];
+#pragma unroll
+        for (int s = 0; s < G::S; ++s)
+#pragma unroll
+            for (int im = 0; im < G::IM; ++im)
+                af[s][im] = bitcast<frag_t>(load_raw<16>(
+                    smem + s * G::PLANE + (pm * G::TM + im * 16 + (lane & 15)) * G::ROWB +
+                    16 * (lane >> 4)));
+        // the wave tile is processed in column groups of JH atoms; a scheduling barrier
+        // between groups keeps their fragments and level sums from being live together
+#pragma unroll
+        for (int h = 0; h < G::JN / G::JH; ++h) {
+#pragma unroll
+            for (int g = G::G; g >= 2; --g) {
+                accv_t T[G::IM][G::JH];
+#pragma unroll
+                for (int im = 0; im < G::IM; ++im)
+#pragma unroll
+                    for (int jh = 0; jh < G::JH; ++jh) T[im][jh] = accv_t{0, 0, 0, 0};
+#pragma unroll
+                for (int s = 1; s <= G::S; ++s) {
+                    const int t = g - s;
+                    if (t < 1 || t > G::S) continue;
+#pragma unroll
+                    for (int jh = 0; jh < G::JH; ++jh) {
+                        const int jn = h * G::JH + jh;
+                        const frag_t bf = bitcast<frag_t>(load_raw<16>(
+                            smem + G::SA_BYTES + (t - 1) * G::PLANE +
+                            (pn * G::TN + jn * 16 + (lane & 15)) * G::ROWB + 16 * (lane >> 4)));
+#pragma unroll
+                        for (int im = 0; im < G::IM; ++im) SAT::mma(T[im][jh], af[s - 1][im], bf);
+                    }
+                }
+                const double w = __builtin_amdgcn_ldexp(1.0, -G::BETA * g);
+#pragma unroll
+                for (int im = 0; im < G::IM; ++im)
+#pragma unroll
+                    for (int jh = 0; jh < G::JH; ++jh)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            acc[im][h * G::JH + jh][v] =
+                                __builtin_fma((double)T[im][jh][v], w, acc[im][h * G::JH + jh][v]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads();  // digit planes are rewritten by the next chunk
+    }
+
+    // ---- 4. rescale and store ----
+#pragma unroll
+    for (int im = 0; im < G::IM; ++im)
+#pragma unroll
+        for (int jn = 0; jn < G::JN; ++jn) {
+            const int col = pn * G::TN + jn * 16 + (lane & 15);
+            const int ecol = expo[G::BM + col];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = pm * G::TM + im * 16 + 4 * (lane >> 4) + v;
+                double val = __builtin_amdgcn_ldexp(acc[im][jn][v], expo[row] + ecol);
+                double* dst = C + row * ldc + col;
+                val *= args.alpha;
+                if (args.beta != 0.0) val += args.beta * (*dst);
+                *dst = val;
+            }
+        }
+}
+
+template <class SAT, class G>
+hipError_t launch_split(const SplitArgs& a, int64_t count, hipStream_t s) {
+    hipLaunchKernelGGL((split_gemm_kernel<SAT, G>), dim3((unsigned)count), dim3(G::NT), 0, s, a);
+    return hipGetLastError();
+}
+
+typedef hipError_t (*split_launch_fn)(const SplitArgs&, int64_t, hipStream_t);
+
+struct SplitInstance {
+    int prec, m, n, k, slices, pm, pn, threads, lds;
+    split_launch_fn launch;
+};
+
+template <class SAT, int PREC, int BM, int BN, int BK, int PM, int PN, int BETA, int S, int G>
+static void add_split(std::vector<SplitInstance>& v) {
+    using Geo = SplitGeom<SAT, BM, BN, BK, PM, PN, BETA, S, G>;
+    v.push_back(SplitInstance{PREC, BM, BN, BK, S, PM, PN, Geo::NT, Geo::LDS,
+                              &launch_split<SAT, Geo>});
+}
+
+static const std::vector<SplitInstance>& split_registry() {
+    static const std::vector<SplitInstance> v = [] {
+        std::vector<SplitInstance> r;
+        constexpr int F16 = KAMI_PREC_F64_SPLIT_F16, BF = KAMI_PREC_F64_SPLIT_BF16;
+        // FP16 digits: 9 bits; default 6 slices (levels <= 7, 21 products)
+        add_split<AtomF16x32, F16, 128, 128, 128, 4, 2, 9, 6, 7>(r);
+        add_split<AtomF16x32, F16, 128, 128, 128, 4, 2, 9, 7, 8>(r);
+        add_split<AtomF16x32, F16, 64, 64, 64, 2, 2, 9, 6, 7>(r);
+        // BF16 digits: 8 bits; default 7 slices (levels <= 8, 28 products)
+        add_split<AtomBF16x32, BF, 128, 128, 128, 4, 2, 8, 7, 8>(r);
+        add_split<AtomBF16x32, BF, 64, 64, 64, 2, 2, 8, 7, 8>(r);
+        return r;
+    }();
+    return v;
+}
+
+static const SplitInstance* resolve_split(const kami_gemm_desc* d) {
+    const int want = d->slices ? d->slices : (d->precision == KAMI_PREC_F64_SPLIT_F16 ? 6 : 7);
+    for (const auto& in : split_registry())
+        if (in.prec == d->precision && in.m == d->m && in.n == d->n && in.k == d->k &&
+            in.slices == want)
+            return &in;
+    return nullptr;
+}
+
+kami_status split_query(const kami_gemm_desc* d, kami_gemm_config* out) {
+    if (d->b_layout != KAMI_B_NK || d->partition == KAMI_PART_3D || d->iters != 1)
+        return KAMI_ENOTSUP;
+    const SplitInstance* in = resolve_split(d);
+    if (!in) return KAMI_ENOTSUP;
+    out->partition = KAMI_PART_2D;
+    out->ranks = in->pm;
+    out->waves_m = in->pm;
+    out->waves_n = in->pn;
+    out->waves_k = 1;
+    out->threads = in->threads;
+    out->k_stage = 32;
+    out->lds_bytes = in->lds;
+    out->slices = in->slices;
+    out->instance = (int32_t)(in - split_registry().data());
+    return KAMI_OK;
+}
+
+kami_status split_launch(const kami_gemm_desc* d, const BlockArgs& b, int64_t count,
                          hipStream_t stream) {
-    (void)d;
-    (void)args;
-    (void)count;
-    (void)stream;
-    return KAMI_ENOTSUP;
+    kami_gemm_config cfg;
+    kami_status st = split_query(d, &cfg);
+    if (st != KAMI_OK) return st;
+    const SplitInstance* in = resolve_split(d);
+    SplitArgs a;
+    a.A = reinterpret_cast<const double*>(b.A);
+    a.B = reinterpret_cast<const double*>(b.B);
+    a.C = reinterpret_cast<double*>(b.C);
+    a.sa = b.sa;
+    a.sb = b.sb;
+    a.sc = b.sc;
+    a.lda = b.lda;
+    a.ldb = b.ldb;
+    a.ldc = b.ldc;
+    a.alpha = b.alpha;
+    a.beta = b.beta;
+    hipError_t e = in->launch(a, count, stream);
+    if (e != hipSuccess) return hip_fail(e);
+    return KAMI_OK;
 }
 
 }  // namespace kami

@@ -98,3 +98,13 @@ def test_device_check_without_gpu_reports_nodev():
     if torch.cuda.is_available():
         pytest.skip("a GPU is present")
     assert K.lib().kami_device_check() == _abi.ENODEV
+
+
+def test_split_precision_query_and_enotsup():
+    cfg = K.query(128, 128, 128, K.PREC_F64_SPLIT_F16)
+    assert cfg["slices"] == 6 and cfg["threads"] == 512 and cfg["lds_bytes"] <= 160 * 1024
+    assert K.query(128, 128, 128, K.PREC_F64_SPLIT_BF16)["slices"] == 7
+    for kw in [dict(b_layout="kn"), dict(partition="3d"), dict(iters=2), dict(slices=3)]:
+        with pytest.raises(K.KamiError) as e:
+            K.query(128, 128, 128, K.PREC_F64_SPLIT_F16, **kw)
+        assert e.value.status == _abi.ENOTSUP

@@ -1,0 +1,136 @@
+"""GPU parity of split-precision FP64 (FP16 / BF16 digit slices on MFMA).
+
+The oracle is the compensated Dot2 product (twice-working-precision accurate).  The
+error bound stated for the split path is normwise per problem:
+    max_ij |C - C*| / max_ij |C*|  <= 1e-13,
+and integer-valued inputs (the reference's ramp and rand_r % 3 known answers) must be
+reproduced exactly, since every digit, digit product and level sum is then exact.
+"""
+import numpy as np
+import pytest
+
+import kami_amd as K
+import oracle
+from kami_amd import _abi
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+SPLITS = [(K.PREC_F64_SPLIT_F16, 6), (K.PREC_F64_SPLIT_F16, 7), (K.PREC_F64_SPLIT_BF16, 7)]
+SIDS = ["f16-s6", "f16-s7", "bf16-s7"]
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print(f"\n[case] {request.node.nodeid}", flush=True)
+
+
+def run(A, B, prec, slices, dev, C=None, **kw):
+    Ad = torch.from_numpy(np.ascontiguousarray(A)).to(dev)
+    Bd = torch.from_numpy(np.ascontiguousarray(B)).to(dev)
+    Cd = None if C is None else torch.from_numpy(np.ascontiguousarray(C)).to(dev)
+    out = K.block_gemm(Ad, Bd, Cd, precision=prec, slices=slices, **kw)
+    return out.cpu().numpy()
+
+
+def normwise(C, exact):
+    d = np.max(np.abs(C - exact), axis=(1, 2))
+    return float(np.max(d / np.maximum(np.max(np.abs(exact), axis=(1, 2)), 1e-300)))
+
+
+def sizes_for(prec, slices):
+    out = []
+    for s in (64, 128):
+        try:
+            K.query(s, s, s, prec, slices=slices)
+            out.append(s)
+        except K.KamiError:
+            pass
+    return out
+
+
+@pytest.mark.parametrize("prec,slices", SPLITS, ids=SIDS)
+def test_ramp_kat_exact(prec, slices, dev):
+    for s in sizes_for(prec, slices):
+        A = np.tile(oracle.ramp(s * s).reshape(1, s, s), (2, 1, 1))
+        C = run(A, A, prec, slices, dev)
+        np.testing.assert_array_equal(C, oracle.gemm_f64(A, A))
+
+
+@pytest.mark.parametrize("prec,slices", SPLITS, ids=SIDS)
+def test_randr_kat_exact(prec, slices, dev):
+    for s in sizes_for(prec, slices):
+        count = 9
+        A = oracle.randr_mod3(count * s * s).reshape(count, s, s) - 1.0
+        B = oracle.randr_mod3(2 * count * s * s)[count * s * s:].reshape(count, s, s) * 3.0
+        C = run(A, B, prec, slices, dev)
+        np.testing.assert_array_equal(C, oracle.gemm_f64(A, B))
+
+
+def log_uniform(rng, shape):
+    return np.where(rng.random(shape) < 0.5, -1.0, 1.0) * 10.0 ** rng.uniform(-4, 4, shape)
+
+
+@pytest.mark.parametrize("prec,slices", SPLITS, ids=SIDS)
+def test_log_uniform_normwise_1e13(prec, slices, dev):
+    """The split-precision workload: sign-random entries, log10|x| ~ U[-4, 4]."""
+    rng = np.random.default_rng(0x4B414D49 + 3)
+    for s in sizes_for(prec, slices):
+        count = 96
+        A = log_uniform(rng, (count, s, s))
+        B = log_uniform(rng, (count, s, s))
+        C = run(A, B, prec, slices, dev)
+        exact, absC = oracle.gemm_dot2(A, B)
+        err = normwise(C, exact)
+        print(f"  {s}^3 normwise {err:.3e}  componentwise "
+              f"{np.max(np.abs(C - exact) / absC):.3e}", flush=True)
+        assert err <= 1e-13
+        Cn = run(A, B, K.PREC_F64, 0, dev)
+        assert normwise(Cn, exact) <= 1e-14
+
+
+@pytest.mark.parametrize("prec,slices", SPLITS, ids=SIDS)
+def test_adversarial_rows(prec, slices, dev):
+    """Zero rows, a single huge entry among tiny ones, signed zeros, exact powers of two,
+    entries 2^-40 below their row maximum, all-equal rows and cancelling pairs."""
+    rng = np.random.default_rng(17)
+    s = 128 if 128 in sizes_for(prec, slices) else 64
+    count = 4
+    A = rng.uniform(-1, 1, (count, s, s))
+    B = rng.uniform(-1, 1, (count, s, s))
+    A[0, 0, :] = 0.0                                  # zero row
+    A[0, 1, :] = -0.0                                 # negative zeros
+    A[0, 2, :] = 1e-6 * rng.uniform(-1, 1, s)
+    A[0, 2, 5] = 1e6                                  # one dominant entry
+    A[0, 3, :] = 2.0 ** rng.integers(-30, 30, s)      # powers of two
+    A[1, 4, :] = 0.75                                 # all equal
+    A[1, 5, :] = 2.0 ** -40 * rng.uniform(-1, 1, s)
+    A[1, 5, 0] = 1.0                                  # entries 2^-40 below the row max
+    B[2, 7, :] = 1.0
+    B[2, 8, :] = -1.0                                 # cancelling columns
+    B[3] *= 1e200                                     # large exponents
+    A[3] *= 1e-200                                    # small exponents
+    C = run(A, B, prec, slices, dev)
+    exact, _ = oracle.gemm_dot2(A, B)
+    assert normwise(C, exact) <= 1e-13
+    assert np.all(C[0, 0] == 0.0) and np.all(C[0, 1] == 0.0)
+
+
+@pytest.mark.parametrize("prec,slices", SPLITS[:1] + SPLITS[2:], ids=[SIDS[0], SIDS[2]])
+def test_alpha_beta(prec, slices, dev):
+    s = 64
+    rng = np.random.default_rng(3)
+    A = rng.integers(-4, 5, (3, s, s)).astype(np.float64)
+    B = rng.integers(-4, 5, (3, s, s)).astype(np.float64)
+    C0 = rng.integers(-9, 10, (3, s, s)).astype(np.float64)
+    C = run(A, B, prec, slices, dev, C=C0, alpha=-0.5, beta=2.0)
+    np.testing.assert_array_equal(C, -0.5 * oracle.gemm_f64(A, B) + 2.0 * C0)
+
+
+def test_split_unsupported_requests_are_enotsup():
+    for kw in [dict(b_layout="kn"), dict(partition="3d"), dict(iters=2), dict(slices=3)]:
+        with pytest.raises(K.KamiError) as e:
+            K.query(128, 128, 128, K.PREC_F64_SPLIT_F16, **kw)
+        assert e.value.status == _abi.ENOTSUP
+    with pytest.raises(K.KamiError):
+        K.query(96, 96, 96, K.PREC_F64_SPLIT_F16)
