@@ -155,6 +155,15 @@ kami_status kami_block_gemm(const kami_gemm_desc* d, const void* A, const void* 
                             int64_t count, int64_t stride_a, int64_t stride_b,
                             int64_t stride_c, void* stream);
 
+/* Times every compiled (partition, P) instance of the descriptor's shape on the given
+ * buffers (C is overwritten; beta is ignored while timing) and makes the fastest the
+ * KAMI_PART_AUTO choice for that shape in this process.  No reference counterpart:
+ * the reference picks the partition offline in its sweep scripts
+ * (scripts/fp64_block_square_GH200.sh:11-25). */
+kami_status kami_block_gemm_autotune(const kami_gemm_desc* d, const void* A, const void* B,
+                                     void* C, int64_t count, void* stream,
+                                     kami_gemm_config* chosen);
+
 /* ---------------------------------------------------------------- batched GEMM */
 /* precision: KAMI_PREC_F64 or KAMI_PREC_F16_F32 or KAMI_PREC_BF16_F32.
  * Builds the work queue, autotunes per size bucket (runs kernels on `stream` when

@@ -115,3 +115,71 @@ def block_gemm(A, B, C=None, *, precision=None, partition="auto", ranks=0, b_lay
                                count, 0, 0, 0, _stream_ptr(stream))
     check(st, f"kami_block_gemm({M}x{N}x{K}, prec={prec}, part={partition}, ranks={ranks})")
     return C[0] if squeeze else C
+
+
+class BatchedPlan:
+    """A batched GEMM plan over problems of uneven size (kami_batched_plan_*).
+
+    problems: sequence of (A, B, C) contiguous 2-D device tensors, A (M, K), B (N, K)
+    ("nk") or (K, N) ("kn"), C (M, N); M, N, K multiples of 16.  The tensors must stay
+    alive while the plan exists.  execute() is one persistent launch, replayed through a
+    hipGraph when use_graph is set.
+    """
+
+    def __init__(self, problems, *, precision=None, b_layout="nk", autotune=False,
+                 use_graph=True, alpha=1.0, beta=0.0, stream=None):
+        if not problems:
+            raise ValueError("empty problem list")
+        A0 = problems[0][0]
+        prec = _precision_of(A0, precision)
+        lay = _layout(b_layout)
+        arr = (_abi.Problem * len(problems))()
+        for i, (A, B, C) in enumerate(problems):
+            m, k = A.shape
+            n = B.shape[0] if lay == B_NK else B.shape[1]
+            p = arr[i]
+            p.m, p.n, p.k = m, n, k
+            p.lda = p.ldb = p.ldc = 0
+            p.A, p.B, p.C = A.data_ptr(), B.data_ptr(), C.data_ptr()
+        opts = _abi.BatchedOpts()
+        opts.b_layout = lay
+        opts.autotune = int(autotune)
+        opts.use_graph = int(use_graph)
+        opts.alpha, opts.beta = alpha, beta
+        handle = ctypes.c_void_p()
+        check(lib().kami_batched_plan_create(arr, len(problems), prec, ctypes.byref(opts),
+                                             _stream_ptr(stream), ctypes.byref(handle)),
+              "kami_batched_plan_create")
+        self._h = handle
+        self._keep = problems
+        self.precision = prec
+
+    def execute(self, stream=None):
+        check(lib().kami_batched_execute(self._h, _stream_ptr(stream)), "kami_batched_execute")
+
+    def info(self) -> dict:
+        info = _abi.PlanInfo()
+        check(lib().kami_batched_plan_info(self._h, ctypes.byref(info)), "kami_batched_plan_info")
+        return {"problems": info.problems, "work_items": info.work_items, "buckets": info.buckets,
+                "workgroups": info.workgroups, "flops": info.flops,
+                "variant_of_bucket": list(info.variant_of_bucket)[: min(16, info.buckets)]}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().kami_batched_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+__all__ += ["BatchedPlan"]

@@ -155,16 +155,42 @@ kami_status validate_desc(const kami_gemm_desc* d) {
     return KAMI_OK;
 }
 
+// Streaming-mode winners of the instance sweep on MI355X (one problem per workgroup,
+// 4096 FP64 / 8192 FP16 problems); used for KAMI_PART_AUTO until
+// kami_block_gemm_autotune has measured the shape in this process.
+struct Prior {
+    int prec, m, n, k, part, ranks;
+};
+static const Prior kPriors[] = {
+    {KAMI_PREC_F64, 16, 16, 16, KAMI_PART_2D, 1},   {KAMI_PREC_F64, 32, 32, 32, KAMI_PART_3D, 2},
+    {KAMI_PREC_F64, 64, 64, 64, KAMI_PART_1D, 4},   {KAMI_PREC_F64, 128, 128, 128, KAMI_PART_3D, 2},
+    {KAMI_PREC_F16_F32, 32, 32, 32, KAMI_PART_2D, 1}, {KAMI_PREC_F16_F32, 64, 64, 64, KAMI_PART_2D, 4},
+    {KAMI_PREC_F16_F32, 128, 128, 128, KAMI_PART_1D, 4},
+    {KAMI_PREC_F16_F32, 256, 256, 256, KAMI_PART_1D, 16},
+    {KAMI_PREC_F16_F32, 256, 256, 16, KAMI_PART_1D, 16},
+    {KAMI_PREC_F16_F32, 256, 256, 32, KAMI_PART_1D, 16},
+    {KAMI_PREC_BF16_F32, 32, 32, 32, KAMI_PART_2D, 1}, {KAMI_PREC_BF16_F32, 64, 64, 64, KAMI_PART_2D, 4},
+    {KAMI_PREC_BF16_F32, 128, 128, 128, KAMI_PART_1D, 4},
+    {KAMI_PREC_BF16_F32, 256, 256, 256, KAMI_PART_1D, 16},
+    {KAMI_PREC_BF16_F32, 256, 256, 16, KAMI_PART_1D, 16},
+    {KAMI_PREC_BF16_F32, 256, 256, 32, KAMI_PART_1D, 16},
+};
+
 const BlockInstance* resolve_block(const kami_gemm_desc* d, kami_status* st) {
     int part = d->partition, ranks = d->ranks;
     if (part == KAMI_PART_AUTO) {
         TuneVal tv;
+        part = KAMI_PART_1D;
+        ranks = 0;
         if (get_tuned_choice(d->precision, d->m, d->n, d->k, &tv)) {
             part = tv.part;
             ranks = tv.ranks;
         } else {
-            part = KAMI_PART_1D;
-            ranks = 0;
+            for (const Prior& p : kPriors)
+                if (p.prec == d->precision && p.m == d->m && p.n == d->n && p.k == d->k) {
+                    part = p.part;
+                    ranks = p.ranks;
+                }
         }
     }
     if (ranks == 0) ranks = default_ranks(d->precision, part, d->m, d->n, d->k);
@@ -290,6 +316,61 @@ kami_status kami_block_gemm(const kami_gemm_desc* d, const void* A, const void* 
     if (st != KAMI_OK) return st;
     hipError_t e = in->launch(args, count, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e);
+    return KAMI_OK;
+}
+
+kami_status kami_block_gemm_autotune(const kami_gemm_desc* d, const void* A, const void* B,
+                                     void* C, int64_t count, void* stream,
+                                     kami_gemm_config* chosen) {
+    kami_status st = validate_desc(d);
+    if (st != KAMI_OK) return st;
+    if (is_split(d->precision)) return KAMI_ENOTSUP;
+    if (count <= 0 || count > 0x7fffffffLL || !A || !B || !C) return KAMI_EINVAL;
+    st = device_check();
+    if (st != KAMI_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    hipError_t he = hipEventCreate(&e0);
+    if (he != hipSuccess) return hip_fail(he);
+    he = hipEventCreate(&e1);
+    if (he != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(he);
+    }
+    float best_ms = 1e30f;
+    int best_part = 0, best_ranks = 0;
+    st = KAMI_ENOTSUP;
+    for (const auto& in : registry()) {
+        if (in.prec != d->precision || in.m != d->m || in.n != d->n || in.k != d->k) continue;
+        kami_gemm_desc t = *d;
+        t.partition = in.part;
+        t.ranks = in.ranks;
+        t.beta = 0.0;  // repeated runs must not feed back into C
+        kami_status r = kami_block_gemm(&t, A, B, C, count, 0, 0, 0, stream);  // warm-up
+        if (r != KAMI_OK) { st = r; break; }
+        (void)hipEventRecord(e0, s);
+        for (int rep = 0; rep < 3; ++rep) (void)kami_block_gemm(&t, A, B, C, count, 0, 0, 0, stream);
+        (void)hipEventRecord(e1, s);
+        he = hipEventSynchronize(e1);
+        if (he != hipSuccess) { st = hip_fail(he); break; }
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_part = in.part;
+            best_ranks = in.ranks;
+        }
+        st = KAMI_OK;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != KAMI_OK) return st;
+    set_tuned_choice(d->precision, d->m, d->n, d->k, best_part, best_ranks);
+    if (chosen) {
+        kami_gemm_desc t = *d;
+        t.partition = KAMI_PART_AUTO;
+        return kami_block_gemm_query(&t, chosen);
+    }
     return KAMI_OK;
 }
 

@@ -127,17 +127,47 @@ split_gemm_kernel(SplitArgs args) {
     double* __restrict__ C = args.C + prob * args.sc;
     const int64_t lda = args.lda, ldb = args.ldb, ldc = args.ldc;
 
-    // ---- 1. row exponents of A and B (one wave per row, lanes across K) ----
-    for (int r = wave; r < G::BM + G::BN; r += G::WAVES) {
-        const double* row = r < G::BM ? A + r * lda : B + (r - G::BM) * ldb;
-        int e = -4096;
-        for (int kp = lane; kp < G::BK / 2; kp += 64) {
-            const d2_t x = bitcast<d2_t>(load_raw<16>(row + 2 * kp));
-            e = max(e, max(exp_of(x[0]), exp_of(x[1])));
+    // ---- 1. row exponents of A and B.  Thread t owns the 16-byte pieces
+    //      p = t + i * NT of every K chunk (row p / 16, k pair p % 16), so the 16 lanes
+    //      of a row reduce with four shuffles; every chunk's loads are independent. ----
+    int ea[G::PA], eb[G::PB];
+#pragma unroll
+    for (int i = 0; i < G::PA; ++i) ea[i] = -4096;
+#pragma unroll
+    for (int i = 0; i < G::PB; ++i) eb[i] = -4096;
+#pragma unroll
+    for (int c = 0; c < G::NCH; ++c) {
+#pragma unroll
+        for (int i = 0; i < G::PA; ++i) {
+            const int p = tid + i * G::NT, row = p >> 4, kp = p & 15;
+            const d2_t x = bitcast<d2_t>(load_raw<16>(A + row * lda + c * G::KC + 2 * kp));
+            ea[i] = max(ea[i], max(exp_of(x[0]), exp_of(x[1])));
         }
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) e = max(e, __shfl_xor(e, off));
-        if (lane == 0) expo[r] = (e == -4096) ? 0 : e;
+        for (int i = 0; i < G::PB; ++i) {
+            const int p = tid + i * G::NT, row = p >> 4, kp = p & 15;
+            const d2_t x = bitcast<d2_t>(load_raw<16>(B + row * ldb + c * G::KC + 2 * kp));
+            eb[i] = max(eb[i], max(exp_of(x[0]), exp_of(x[1])));
+        }
+    }
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+#pragma unroll
+        for (int i = 0; i < G::PA; ++i) ea[i] = max(ea[i], __shfl_xor(ea[i], off));
+#pragma unroll
+        for (int i = 0; i < G::PB; ++i) eb[i] = max(eb[i], __shfl_xor(eb[i], off));
+    }
+#pragma unroll
+    for (int i = 0; i < G::PA; ++i) {
+        if (ea[i] == -4096) ea[i] = 0;
+        const int p = tid + i * G::NT;
+        if ((p & 15) == 0) expo[p >> 4] = ea[i];
+    }
+#pragma unroll
+    for (int i = 0; i < G::PB; ++i) {
+        if (eb[i] == -4096) eb[i] = 0;
+        const int p = tid + i * G::NT;
+        if ((p & 15) == 0) expo[G::BM + (p >> 4)] = eb[i];
     }
 
     double acc[G::IM][G::JN][4];
@@ -178,21 +208,22 @@ split_gemm_kernel(SplitArgs args) {
         }
     };
 
+    load_chunk(0);
     __syncthreads();  // exponents visible
 
     for (int c = 0; c < G::NCH; ++c) {
-        load_chunk(c);
         // ---- 2. slice the chunk into digit planes ----
 #pragma unroll
         for (int i = 0; i < G::PA; ++i) {
             const int p = tid + i * G::NT, row = p >> 4, kp = p & 15;
-            slice_piece(xa[i], row, kp, expo[row], smem);
+            slice_piece(xa[i], row, kp, ea[i], smem);
         }
 #pragma unroll
         for (int i = 0; i < G::PB; ++i) {
             const int p = tid + i * G::NT, row = p >> 4, kp = p & 15;
-            slice_piece(xb[i], row, kp, expo[G::BM + row], smem + G::SA_BYTES);
+            slice_piece(xb[i], row, kp, eb[i], smem + G::SA_BYTES);
         }
+        if (c + 1 < G::NCH) load_chunk(c + 1);  // in flight during the products
         __syncthreads();
 
         // ---- 3. level products on MFMA, drained to FP64 ----

@@ -108,3 +108,30 @@ def test_split_precision_query_and_enotsup():
         with pytest.raises(K.KamiError) as e:
             K.query(128, 128, 128, K.PREC_F64_SPLIT_F16, **kw)
         assert e.value.status == _abi.ENOTSUP
+
+
+def test_batched_plan_validation_without_device():
+    arr = (_abi.Problem * 2)()
+    for p in arr:
+        p.m, p.n, p.k = 32, 32, 32
+        p.A = p.B = p.C = 1 << 20
+    opts = _abi.BatchedOpts()
+    opts.alpha = 1.0
+    h = ctypes.c_void_p()
+    lib = K.lib()
+    arr[1].k = 24  # not a multiple of 16
+    assert lib.kami_batched_plan_create(arr, 2, K.PREC_F64, ctypes.byref(opts), None,
+                                        ctypes.byref(h)) == _abi.EINVAL
+    arr[1].k = 32
+    arr[0].A = None
+    assert lib.kami_batched_plan_create(arr, 2, K.PREC_F64, ctypes.byref(opts), None,
+                                        ctypes.byref(h)) == _abi.EINVAL
+    arr[0].A = 1 << 20
+    opts.autotune, opts.beta = 1, 1.0  # tuning overwrites C: refused with beta != 0
+    assert lib.kami_batched_plan_create(arr, 2, K.PREC_F64, ctypes.byref(opts), None,
+                                        ctypes.byref(h)) == _abi.EINVAL
+    opts.autotune, opts.beta = 0, 0.0
+    assert lib.kami_batched_plan_create(arr, 2, K.PREC_F64_SPLIT_F16, ctypes.byref(opts), None,
+                                        ctypes.byref(h)) == _abi.ENOTSUP
+    assert lib.kami_batched_plan_create(arr, 0, K.PREC_F64, ctypes.byref(opts), None,
+                                        ctypes.byref(h)) == _abi.EINVAL
