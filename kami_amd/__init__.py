@@ -183,3 +183,81 @@ class BatchedPlan:
 
 
 __all__ += ["BatchedPlan"]
+
+
+class BsrMatrix:
+    """Block-sparse row matrix on the device: mb x nb blocks of bs x bs (row-major).
+
+    rowptr: int32 (mb + 1,), colidx: int32 (nnzb,) ascending per row, vals: (nnzb, bs, bs).
+    """
+
+    def __init__(self, mb, nb, bs, rowptr, colidx, vals):
+        self.mb, self.nb, self.bs = int(mb), int(nb), int(bs)
+        self.rowptr, self.colidx, self.vals = rowptr, colidx, vals
+
+    @property
+    def nnzb(self):
+        return int(self.colidx.numel())
+
+    def _c(self):
+        b = _abi.Bsr()
+        b.mb, b.nb, b.bs, b.nnzb = self.mb, self.nb, self.bs, self.nnzb
+        b.rowptr = self.rowptr.data_ptr()
+        b.colidx = self.colidx.data_ptr() if self.nnzb else None
+        b.vals = self.vals.data_ptr() if self.nnzb else None
+        return b
+
+
+def spgemm_symbolic(A: BsrMatrix, B: BsrMatrix, stream=None):
+    """Row pointer of C = A * B (exclusive scan of per-row block counts) and nnzb(C).
+    Synchronises the stream to read nnzb(C) back, as the reference does
+    (src/sparse/spgemm/2d/block_spgemm_2d_half_mma.cu:808)."""
+    import torch
+    rowptr = torch.empty(A.mb + 1, dtype=torch.int32, device=A.rowptr.device)
+    nnz = ctypes.c_int64()
+    a, b = A._c(), B._c()
+    check(lib().kami_bsr_spgemm_symbolic(ctypes.byref(a), ctypes.byref(b),
+                                         ctypes.c_void_p(rowptr.data_ptr()), ctypes.byref(nnz),
+                                         None, 0, _stream_ptr(stream)), "kami_bsr_spgemm_symbolic")
+    return rowptr, int(nnz.value)
+
+
+def spgemm_fill(A: BsrMatrix, B: BsrMatrix, rowptrC, nnzbC, stream=None):
+    """Ascending block columns of every row of C."""
+    import torch
+    colidx = torch.empty(max(nnzbC, 1), dtype=torch.int32, device=A.rowptr.device)
+    a, b = A._c(), B._c()
+    check(lib().kami_bsr_spgemm_fill(ctypes.byref(a), ctypes.byref(b),
+                                     ctypes.c_void_p(rowptrC.data_ptr()),
+                                     ctypes.c_void_p(colidx.data_ptr()), None, 0,
+                                     _stream_ptr(stream)), "kami_bsr_spgemm_fill")
+    return colidx[:nnzbC]
+
+
+def spgemm_numeric(A: BsrMatrix, B: BsrMatrix, rowptrC, colidxC, *, precision=None, alpha=1.0,
+                   stream=None):
+    """Values of C = alpha * A * B on the symbolic structure: (nnzbC, bs, bs), float32 for
+    FP16/BF16 inputs, float64 for FP64."""
+    import torch
+    prec = _precision_of(A.vals, precision)
+    out_dt = torch.float64 if prec == PREC_F64 else torch.float32
+    nnz = int(colidxC.numel())
+    vals = torch.empty((max(nnz, 1), A.bs, A.bs), dtype=out_dt, device=A.rowptr.device)
+    a, b = A._c(), B._c()
+    check(lib().kami_bsr_spgemm_numeric(ctypes.byref(a), ctypes.byref(b),
+                                        ctypes.c_void_p(rowptrC.data_ptr()),
+                                        ctypes.c_void_p(colidxC.data_ptr() if nnz else 0), nnz,
+                                        ctypes.c_void_p(vals.data_ptr()), prec, alpha,
+                                        _stream_ptr(stream)), "kami_bsr_spgemm_numeric")
+    return vals[:nnz]
+
+
+def bsr_spgemm(A: BsrMatrix, B: BsrMatrix, *, precision=None, alpha=1.0, stream=None):
+    """C = alpha * A * B as a BsrMatrix (symbolic, fill and numeric phases)."""
+    rowptr, nnz = spgemm_symbolic(A, B, stream)
+    colidx = spgemm_fill(A, B, rowptr, nnz, stream)
+    vals = spgemm_numeric(A, B, rowptr, colidx, precision=precision, alpha=alpha, stream=stream)
+    return BsrMatrix(A.mb, B.nb, A.bs, rowptr, colidx, vals)
+
+
+__all__ += ["BsrMatrix", "spgemm_symbolic", "spgemm_fill", "spgemm_numeric", "bsr_spgemm"]
