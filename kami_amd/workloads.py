@@ -1,0 +1,217 @@
+"""Synthetic workloads of the block-GEMM path and their GPU measurements.
+
+Five workloads (seed 0x4B414D49 + index, numpy PCG64 on the host or torch's generator
+on the device):
+  w1  FP64 block GEMM, M = N = K = 64, 1024 problems, U[-1, 1], 2D partition;
+  w2  FP16 in / FP32 accumulate, M = N = K in {32, 64, 128, 256} and 256 x 256 x {16, 32},
+      8192 problems per shape, N(0, 1) cast to FP16, every partition timed;
+  w3  FP64 through FP16 slices, M = N = K = 128, 4096 problems, sign-random entries with
+      log10|x| ~ U[-4, 4], against native FP64;
+  w4  batched uneven sizes: 20000 problems, M, N, K log-uniform in [16, 256] rounded to
+      multiples of 16, U[-1, 1], FP16->FP32 and FP64, one persistent launch per
+      execution in a hipGraph, autotuned per size bucket;
+  w5  block SpGEMM C = A * A, 8192 x 8192 with 32 x 32 blocks at 2 % block density,
+      N(0, 1) values in FP16, FP32 accumulation.
+Timing: inputs resident in HBM, warm-up calls, then torch.cuda events on the stream the
+library launches on.  FLOP counts are algorithmic: 2 M N K per problem (FP64-equivalent
+for split precision) and 2 * nnzCub * bs^3 for SpGEMM (the reference's accounting,
+src/sparse/spgemm/1d/block_spgemm_1d_half_mma.cu:513).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import kami_amd as K
+
+SEED = 0x4B414D49
+PEAK = {"f64": 78.6, "f16": 2500.0, "hbm_gbs": 8000.0}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def time_ms(fn, steps=10, warmup=2):
+    torch = _torch()
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(steps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / steps
+
+
+def log_uniform_signed(rng, shape, lo=-4.0, hi=4.0):
+    return np.where(rng.random(shape) < 0.5, -1.0, 1.0) * 10.0 ** rng.uniform(lo, hi, shape)
+
+
+# ------------------------------------------------------------------ w1
+
+def run_w1(steps=20, warmup=3):
+    torch = _torch()
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(SEED + 0)
+    m = n = k = 64
+    count = 1024
+    A = torch.from_numpy(rng.uniform(-1, 1, (count, m, k))).to(dev)
+    B = torch.from_numpy(rng.uniform(-1, 1, (count, n, k))).to(dev)
+    C = torch.empty((count, m, n), dtype=torch.float64, device=dev)
+    ms = time_ms(lambda: K.block_gemm(A, B, C, partition="2d"), steps, warmup)
+    flops = 2.0 * m * n * k * count
+    byts = 8.0 * (m * k + n * k + m * n) * count
+    ref = torch.bmm(A[:64], B[:64].transpose(1, 2).contiguous())
+    err = float((C[:64] - ref).abs().max() / ref.abs().max())
+    tf = flops / ms / 1e9
+    return {"workload": "FP64 64x64x64 x1024, U[-1,1], 2D", "ms": ms, "tflops": tf,
+            "frac_f64_peak": tf / PEAK["f64"], "hbm_gbs": byts / ms / 1e6,
+            "note": "100 MB of operands fit the 256 MB last-level cache: MALL-warm",
+            "max_rel_err_vs_fp64": err}
+
+
+# ------------------------------------------------------------------ w2
+
+W2_SHAPES = [(32, 32, 32), (64, 64, 64), (128, 128, 128), (256, 256, 256), (256, 256, 16),
+             (256, 256, 32)]
+
+
+def run_w2(count=8192, steps=10, warmup=2, resident_tflop=4.0):
+    torch = _torch()
+    dev = torch.device("cuda:0")
+    out = {}
+    for (m, n, k) in W2_SHAPES:
+        g = torch.Generator(device=dev)
+        g.manual_seed(SEED + 1 + m * 1000 + k)
+        A = torch.randn((count, m, k), generator=g, device=dev, dtype=torch.float32).half()
+        B = torch.randn((count, n, k), generator=g, device=dev, dtype=torch.float32).half()
+        C = torch.empty((count, m, n), dtype=torch.float32, device=dev)
+        ref = torch.bmm(A[:16].double(), B[:16].double().transpose(1, 2).contiguous())
+        flops = 2.0 * m * n * k * count
+        byts = (2.0 * (m * k + n * k) + 4.0 * m * n) * count
+        cres = min(count, 2048)
+        it_res = max(1, int(round(resident_tflop * 1e12 / (2.0 * m * n * k * cres))))
+        shape_out = {}
+        for part in ("1d", "2d", "3d"):
+            try:
+                cfg = K.query(m, n, k, K.PREC_F16_F32, partition=part)
+            except K.KamiError:
+                shape_out[part] = {"available": False}
+                continue
+            ms = time_ms(lambda: K.block_gemm(A, B, C, partition=part), steps, warmup)
+            err = float((C[:16].double() - ref).abs().max() / ref.abs().max())
+            msr = time_ms(lambda: K.block_gemm(A[:cres], B[:cres], C[:cres], partition=part,
+                                               iters=it_res), 3, 1)
+            tf = flops / ms / 1e9
+            tfr = 2.0 * m * n * k * cres * it_res / msr / 1e9
+            shape_out[part] = {"ranks": cfg["ranks"], "streaming_tflops": tf,
+                               "streaming_hbm_gbs": byts / ms / 1e6,
+                               "resident_tflops": tfr, "resident_frac_f16_peak": tfr / PEAK["f16"],
+                               "max_rel_err_vs_fp64": err}
+        out[f"{m}x{n}x{k}"] = shape_out
+        del A, B, C
+    return out
+
+
+# ------------------------------------------------------------------ w4
+
+def uneven_sizes(rng, count, lo=16, hi=256):
+    x = np.exp(rng.uniform(np.log(lo), np.log(hi), size=(count, 3)))
+    return np.clip(np.round(x / 16) * 16, lo, hi).astype(np.int64)
+
+
+def run_w4(count=20000, steps=10, warmup=2, precisions=("f16", "f64")):
+    torch = _torch()
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(SEED + 3)
+    sizes = uneven_sizes(rng, count)
+    out = {"problems": count}
+    for pname in precisions:
+        in_dt = torch.float16 if pname == "f16" else torch.float64
+        out_dt = torch.float32 if pname == "f16" else torch.float64
+        na = int((sizes[:, 0] * sizes[:, 2]).sum())
+        nb = int((sizes[:, 1] * sizes[:, 2]).sum())
+        nc = int((sizes[:, 0] * sizes[:, 1]).sum())
+        g = torch.Generator(device=dev)
+        g.manual_seed(SEED + 3)
+        Abuf = (torch.rand(na, generator=g, device=dev, dtype=torch.float64) * 2 - 1).to(in_dt)
+        Bbuf = (torch.rand(nb, generator=g, device=dev, dtype=torch.float64) * 2 - 1).to(in_dt)
+        Cbuf = torch.empty(nc, dtype=out_dt, device=dev)
+        probs = []
+        oa = ob = oc = 0
+        for (m, n, k) in sizes:
+            m, n, k = int(m), int(n), int(k)
+            probs.append((Abuf[oa:oa + m * k].view(m, k), Bbuf[ob:ob + n * k].view(n, k),
+                          Cbuf[oc:oc + m * n].view(m, n)))
+            oa += m * k
+            ob += n * k
+            oc += m * n
+        flops = 2.0 * float((sizes[:, 0] * sizes[:, 1] * sizes[:, 2]).sum())
+        elt = 2 if pname == "f16" else 8
+        byts = elt * (na + nb) + (4 if pname == "f16" else 8) * nc
+        with K.BatchedPlan(probs, autotune=True, use_graph=True) as plan:
+            info = plan.info()
+            ms = time_ms(plan.execute, steps, warmup)
+        errs = []
+        for idx in range(0, count, max(1, count // 32)):
+            A, B, C = probs[idx]
+            ref = A.double() @ B.double().t()
+            errs.append(float((C.double() - ref).abs().max() / ref.abs().max()))
+        tf = flops / ms / 1e9
+        peak = PEAK["f16"] if pname == "f16" else PEAK["f64"]
+        out[pname] = {"ms": ms, "tflops": tf, "frac_mfma_peak": tf / peak,
+                      "hbm_gbs": byts / ms / 1e6, "gflop": flops / 1e9,
+                      "work_items": info["work_items"], "buckets": info["buckets"],
+                      "workgroups": info["workgroups"], "max_rel_err_vs_fp64": max(errs)}
+        del probs, Abuf, Bbuf, Cbuf
+    return out
+
+
+# ------------------------------------------------------------------ w5
+
+def spgemm_input(rng, mb=256, bs=32, density=0.02):
+    mask = rng.random((mb, mb)) < density
+    rp = np.concatenate([[0], np.cumsum(mask.sum(1))]).astype(np.int32)
+    ci = np.nonzero(mask)[1].astype(np.int32)
+    vals = rng.standard_normal((len(ci), bs, bs)).astype(np.float16)
+    return rp, ci, vals
+
+
+def run_w5(steps=10, warmup=2):
+    torch = _torch()
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(SEED + 4)
+    mb, bs = 256, 32
+    rp, ci, vals = spgemm_input(rng, mb, bs)
+    A = K.BsrMatrix(mb, mb, bs, torch.from_numpy(rp).to(dev), torch.from_numpy(ci).to(dev),
+                    torch.from_numpy(vals.view(np.int16)).to(dev).view(torch.float16))
+    # nnzCub = sum over A's blocks (i, k) of nnz(B row k), the reference's FLOP unit
+    nnzcub = int(sum(int(rp[c + 1] - rp[c]) for c in ci))
+    rowptr, nnz = K.spgemm_symbolic(A, A)
+    colidx = K.spgemm_fill(A, A, rowptr, nnz)
+
+    def symbolic():
+        r, z = K.spgemm_symbolic(A, A)
+        K.spgemm_fill(A, A, r, z)
+
+    ms_sym = time_ms(symbolic, steps, warmup)
+    Cv = torch.empty((nnz, bs, bs), dtype=torch.float32, device=dev)
+
+    def numeric():
+        a = A._c()
+        import ctypes
+        K.check(K.lib().kami_bsr_spgemm_numeric(
+            ctypes.byref(a), ctypes.byref(a), ctypes.c_void_p(rowptr.data_ptr()),
+            ctypes.c_void_p(colidx.data_ptr()), nnz, ctypes.c_void_p(Cv.data_ptr()),
+            K.PREC_F16_F32, 1.0, K._stream_ptr(None)), "numeric")
+
+    ms_num = time_ms(numeric, steps, warmup)
+    flops = 2.0 * nnzcub * bs ** 3
+    byts = 2.0 * len(ci) * bs * bs + 4.0 * nnz * bs * bs  # A read once (A = B), C written
+    return {"workload": "BSR C=A*A, 8192^2, 32x32 blocks, 2% density, FP16->FP32",
+            "nnzb_A": int(len(ci)), "nnzb_C": int(nnz), "nnzCub": nnzcub,
+            "symbolic_ms": ms_sym, "numeric_ms": ms_num, "numeric_tflops": flops / ms_num / 1e9,
+            "numeric_hbm_gbs": byts / ms_num / 1e6}
