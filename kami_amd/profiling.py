@@ -1,0 +1,70 @@
+"""Small drivers for profiling one kernel under rocprofv3.
+
+    python -m kami_amd.profiling copy --mib 1024         # plain device copy (calibration)
+    python -m kami_amd.profiling w3 --path native        # FP64 128^3 x 4096 native
+    python -m kami_amd.profiling w3 --path split         # same through FP16 slices
+    python -m kami_amd.profiling w1 | w4 --prec f16 | w5
+
+Each run launches the kernel `--reps` times after one warm-up, so per-dispatch counters
+(FETCH_SIZE, WRITE_SIZE in KiB) can be divided by the algorithmic bytes of one launch.
+The copy reads and writes exactly `--mib` MiB per launch: its counters give the scale
+factor between counter KiB and bytes moved.
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+
+import numpy as np
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["copy", "w1", "w3", "w4", "w5"])
+    ap.add_argument("--mib", type=int, default=1024)
+    ap.add_argument("--path", choices=["native", "split"], default="native")
+    ap.add_argument("--prec", choices=["f16", "f64"], default="f16")
+    ap.add_argument("--reps", type=int, default=3)
+    args = ap.parse_args(argv)
+
+    import torch
+
+    import kami_amd as K
+    from kami_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    if args.what == "copy":
+        n = args.mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(args.reps + 1):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"copy: {args.reps + 1} launches of {args.mib} MiB read + {args.mib} MiB written")
+        return
+    if args.what == "w3":
+        rng = np.random.default_rng(W.SEED + 2)
+        m = n = k = 128
+        count = 4096
+        A = torch.from_numpy(W.log_uniform_signed(rng, (count, m, k))).to(dev)
+        B = torch.from_numpy(W.log_uniform_signed(rng, (count, n, k))).to(dev)
+        C = torch.empty((count, m, n), dtype=torch.float64, device=dev)
+        prec = K.PREC_F64 if args.path == "native" else K.PREC_F64_SPLIT_F16
+        for _ in range(args.reps + 1):
+            K.block_gemm(A, B, C, precision=prec)
+        torch.cuda.synchronize()
+        byts = 8 * (m * k + n * k + m * n) * count
+        print(f"w3 {args.path}: {args.reps + 1} launches, {byts} algorithmic bytes each")
+        return
+    if args.what == "w1":
+        W.run_w1(steps=args.reps, warmup=1)
+    elif args.what == "w4":
+        W.run_w4(steps=args.reps, warmup=1, precisions=(args.prec,))
+    elif args.what == "w5":
+        W.run_w5(steps=args.reps, warmup=1)
+    torch.cuda.synchronize()
+    print(f"{args.what}: done", file=sys.stderr)
+
+
+if __name__ == "__main__":
+    main()
