@@ -85,7 +85,9 @@ typedef struct kami_gemm_desc {
     int64_t lda, ldb, ldc; /* elements; 0 = packed */
     double alpha, beta;    /* C = alpha * (A B) + beta * C */
     int32_t slices;        /* split precision: slices per operand, 0 = default */
-    int32_t reserved;
+    int32_t pipeline;      /* 0 = tuned choice; 1 = one workgroup per problem;
+                              2 = persistent workgroups prefetching several K stages
+                              ahead across problems (2D, B_NK, iters = 1 only) */
 } kami_gemm_desc;
 
 /* The instance a descriptor resolves to (filled by kami_block_gemm_query). */
@@ -98,6 +100,8 @@ typedef struct kami_gemm_config {
     int32_t lds_bytes;      /* static LDS of the instance */
     int32_t slices;         /* split precision slices (0 otherwise) */
     int32_t instance;       /* internal instance id */
+    int32_t pipeline;       /* resolved pipeline (1 or 2) */
+    int32_t prefetch;       /* K stages loaded ahead of the MFMAs */
 } kami_gemm_config;
 
 /* One problem of a batched plan. Pointers are device pointers that must stay valid

@@ -40,7 +40,7 @@ def device_ok() -> bool:
 
 
 def make_desc(m, n, k, precision, *, partition="auto", ranks=0, b_layout="nk", iters=1,
-              alpha=1.0, beta=0.0, lda=0, ldb=0, ldc=0, slices=0) -> _abi.GemmDesc:
+              alpha=1.0, beta=0.0, lda=0, ldb=0, ldc=0, slices=0, pipeline=0) -> _abi.GemmDesc:
     d = _abi.GemmDesc()
     check(lib().kami_gemm_desc_init(ctypes.byref(d), m, n, k, precision), "kami_gemm_desc_init")
     d.partition = _part(partition)
@@ -51,6 +51,7 @@ def make_desc(m, n, k, precision, *, partition="auto", ranks=0, b_layout="nk", i
     d.beta = beta
     d.lda, d.ldb, d.ldc = lda, ldb, ldc
     d.slices = slices
+    d.pipeline = pipeline
     return d
 
 
@@ -83,7 +84,7 @@ def _stream_ptr(stream):
 
 
 def block_gemm(A, B, C=None, *, precision=None, partition="auto", ranks=0, b_layout="nk",
-               iters=1, alpha=1.0, beta=0.0, slices=0, stream=None):
+               iters=1, alpha=1.0, beta=0.0, slices=0, pipeline=0, stream=None):
     """C = alpha * A @ op(B) + beta * C for a batch of problems, one per workgroup.
 
     A: (count, M, K) or (M, K); B: (count, N, K) ("nk", the reference layout) or
@@ -109,7 +110,7 @@ def block_gemm(A, B, C=None, *, precision=None, partition="auto", ranks=0, b_lay
     if C.dtype != out_dtype or tuple(C.shape) != (count, M, N):
         raise ValueError(f"C must be {out_dtype} of shape {(count, M, N)}")
     d = make_desc(M, N, K, prec, partition=partition, ranks=ranks, b_layout=lay, iters=iters,
-                  alpha=alpha, beta=beta, slices=slices)
+                  alpha=alpha, beta=beta, slices=slices, pipeline=pipeline)
     st = lib().kami_block_gemm(ctypes.byref(d), ctypes.c_void_p(A.data_ptr()),
                                ctypes.c_void_p(B.data_ptr()), ctypes.c_void_p(C.data_ptr()),
                                count, 0, 0, 0, _stream_ptr(stream))

@@ -35,14 +35,14 @@ class GemmDesc(ctypes.Structure):
                 ("partition", c_int32), ("ranks", c_int32), ("b_layout", c_int32),
                 ("iters", c_int32), ("lda", c_int64), ("ldb", c_int64), ("ldc", c_int64),
                 ("alpha", c_double), ("beta", c_double), ("slices", c_int32),
-                ("reserved", c_int32)]
+                ("pipeline", c_int32)]
 
 
 class GemmConfig(ctypes.Structure):
     _fields_ = [("partition", c_int32), ("ranks", c_int32), ("waves_m", c_int32),
                 ("waves_n", c_int32), ("waves_k", c_int32), ("threads", c_int32),
                 ("k_stage", c_int32), ("lds_bytes", c_int32), ("slices", c_int32),
-                ("instance", c_int32)]
+                ("instance", c_int32), ("pipeline", c_int32), ("prefetch", c_int32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

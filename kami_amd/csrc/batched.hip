@@ -196,7 +196,6 @@ __device__ void tile_gemm(const kami_problem& P, int row0, int col0, unsigned ch
 constexpr int kVariants = 4;
 static const int kVariantTM[kVariants] = {64, 32, 64, 128};
 static const int kVariantTN[kVariants] = {64, 32, 64, 64};
-static const int kVariantPart[kVariants] = {KAMI_PART_2D, KAMI_PART_2D, KAMI_PART_1D, KAMI_PART_2D};
 
 template <class AT>
 struct Variants {

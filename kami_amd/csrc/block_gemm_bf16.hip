@@ -45,6 +45,12 @@ void register_bf16_instances(std::vector<BlockInstance>& v) {
     add_instance<A, F, 256, 256, 32, 8, 1, 1>(v, 1, 8);
     add_instance<A, F, 256, 256, 32, 16, 1, 1>(v, 1, 16);
     add_instance<A, F, 256, 256, 32, 4, 4, 1>(v, 2, 4);
+    // persistent 2D forms (pipeline 2)
+    add_pipe_instance<A, F, 64, 64, 64, 2, 2, 4>(v, 2);
+    add_pipe_instance<A, F, 64, 64, 64, 4, 4, 4>(v, 4);
+    add_pipe_instance<A, F, 128, 128, 128, 2, 2, 4>(v, 2);
+    add_pipe_instance<A, F, 128, 128, 128, 4, 4, 4>(v, 4);
+    add_pipe_instance<H, F, 256, 256, 16, 4, 4, 4>(v, 4);
 }
 
 }  // namespace kami

@@ -32,6 +32,12 @@ void register_f64_instances(std::vector<BlockInstance>& v) {
     add_instance<A, F, 128, 128, 128, 2, 2, 1>(v, 2, 2);
     add_instance<A, F, 128, 128, 128, 4, 4, 1>(v, 2, 4);
     add_instance<A, F, 128, 128, 128, 2, 2, 2>(v, 3, 2);
+    // persistent 2D forms (pipeline 2)
+    add_pipe_instance<A, F, 32, 32, 32, 2, 2, 4>(v, 2);
+    add_pipe_instance<A, F, 64, 64, 64, 2, 2, 4>(v, 2);
+    add_pipe_instance<A, F, 64, 64, 64, 4, 4, 4>(v, 4);
+    add_pipe_instance<A, F, 128, 128, 128, 2, 2, 4>(v, 2);
+    add_pipe_instance<A, F, 128, 128, 128, 4, 4, 2>(v, 4);
 }
 
 }  // namespace kami

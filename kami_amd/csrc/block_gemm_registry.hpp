@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "block_gemm_kernel.hpp"
+#include "block_gemm_pipe.hpp"
 
 namespace kami {
 
@@ -25,6 +26,8 @@ struct BlockInstance {
     int lds;
     int atom_k;     // K of one fragment step
     block_launch_fn launch;
+    int pipe;       // 1: one workgroup per problem; 2: persistent, deep prefetch
+    int prefetch;   // K stages loaded ahead of the MFMAs
 };
 
 // Largest K stage (a divisor of BK, a multiple of PK * atom K step) whose LDS staging
@@ -51,7 +54,38 @@ void add_instance(std::vector<BlockInstance>& v, int part, int ranks) {
     constexpr int KS = choose_ks<AT, BM, BN, BK, PM, PN, PK>(kStageBudget);
     using G = BlockGeom<AT, BM, BN, BK, PM, PN, PK, KS>;
     v.push_back(BlockInstance{PREC, BM, BN, BK, part, ranks, PM, PN, PK, KS, G::NT, G::LDS,
-                              AT::KSTEP, &launch_block_gemm<AT, G>});
+                              AT::KSTEP, &launch_block_gemm<AT, G>, 1, 1});
+}
+
+// Largest prefetch depth <= want that divides the stage count.
+constexpr int pick_depth(int ns, int want) {
+    for (int d = want; d > 1; --d)
+        if (ns % d == 0) return d;
+    return 1;
+}
+
+// Largest K stage whose two LDS buffers fit `budget` (the persistent form always
+// double-buffers, also when one stage covers all of K).
+template <class AT, int BM, int BN, int BK, int PM, int PN>
+constexpr int choose_ks_pipe(int budget) {
+    constexpr int elt = (int)sizeof(typename AT::in_t);
+    int best = 0;
+    for (int ks = BK; ks >= AT::KSTEP; ks /= 2) {
+        if (BK % ks != 0 || ks % AT::KSTEP != 0) continue;
+        best = ks;
+        if (2 * (BM + BN) * (ks * elt + 16) <= budget) return ks;
+    }
+    return best;
+}
+
+// Persistent 2D instance (PM x PN waves) with prefetch depth up to PD_WANT.
+template <class AT, int PREC, int BM, int BN, int BK, int PM, int PN, int PD_WANT>
+void add_pipe_instance(std::vector<BlockInstance>& v, int ranks) {
+    constexpr int KS = choose_ks_pipe<AT, BM, BN, BK, PM, PN>(kStageBudget);
+    using G = BlockGeom<AT, BM, BN, BK, PM, PN, 1, KS>;
+    constexpr int PD = pick_depth(G::NS, PD_WANT);
+    v.push_back(BlockInstance{PREC, BM, BN, BK, 2, ranks, PM, PN, 1, KS, G::NT, 2 * G::STAGE_LDS,
+                              AT::KSTEP, &launch_block_gemm_pipe<AT, G, PD>, 2, PD});
 }
 
 void register_f64_instances(std::vector<BlockInstance>& v);

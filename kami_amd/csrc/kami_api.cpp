@@ -111,18 +111,18 @@ struct TuneKey {
     }
 };
 struct TuneVal {
-    int part, ranks;
+    int part, ranks, pipe;
 };
 static std::vector<std::pair<TuneKey, TuneVal>> g_tuned;
 
-void set_tuned_choice(int prec, int m, int n, int k, int part, int ranks) {
+void set_tuned_choice(int prec, int m, int n, int k, int part, int ranks, int pipe) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (auto& kv : g_tuned)
         if (kv.first == TuneKey{prec, m, n, k}) {
-            kv.second = TuneVal{part, ranks};
+            kv.second = TuneVal{part, ranks, pipe};
             return;
         }
-    g_tuned.push_back({TuneKey{prec, m, n, k}, TuneVal{part, ranks}});
+    g_tuned.push_back({TuneKey{prec, m, n, k}, TuneVal{part, ranks, pipe}});
 }
 
 static bool get_tuned_choice(int prec, int m, int n, int k, TuneVal* out) {
@@ -143,6 +143,7 @@ kami_status validate_desc(const kami_gemm_desc* d) {
     if (d->ranks < 0 || d->iters < 1) return KAMI_EINVAL;
     if (d->b_layout != KAMI_B_NK && d->b_layout != KAMI_B_KN) return KAMI_EINVAL;
     if (d->lda < 0 || d->ldb < 0 || d->ldc < 0 || d->slices < 0) return KAMI_EINVAL;
+    if (d->pipeline < 0 || d->pipeline > 2) return KAMI_EINVAL;
     const int64_t lda = d->lda ? d->lda : d->k;
     const int64_t ldb = d->ldb ? d->ldb : (d->b_layout == KAMI_B_NK ? d->k : d->n);
     const int64_t ldc = d->ldc ? d->ldc : d->n;
@@ -159,7 +160,7 @@ kami_status validate_desc(const kami_gemm_desc* d) {
 // 4096 FP64 / 8192 FP16 problems); used for KAMI_PART_AUTO until
 // kami_block_gemm_autotune has measured the shape in this process.
 struct Prior {
-    int prec, m, n, k, part, ranks;
+    int prec, m, n, k, part, ranks, pipe = 1;
 };
 static const Prior kPriors[] = {
     {KAMI_PREC_F64, 16, 16, 16, KAMI_PART_2D, 1},   {KAMI_PREC_F64, 32, 32, 32, KAMI_PART_3D, 2},
@@ -176,35 +177,57 @@ static const Prior kPriors[] = {
     {KAMI_PREC_BF16_F32, 256, 256, 32, KAMI_PART_1D, 16},
 };
 
+// The persistent form handles the 2D partition with B stored N x K and iters = 1.
+static bool pipe_allowed(const kami_gemm_desc* d) {
+    return d->b_layout == KAMI_B_NK && d->iters == 1;
+}
+
 const BlockInstance* resolve_block(const kami_gemm_desc* d, kami_status* st) {
-    int part = d->partition, ranks = d->ranks;
+    int part = d->partition, ranks = d->ranks, pipe = d->pipeline;
+    if (pipe < 0 || pipe > 2) {
+        *st = KAMI_EINVAL;
+        return nullptr;
+    }
     if (part == KAMI_PART_AUTO) {
         TuneVal tv;
         part = KAMI_PART_1D;
         ranks = 0;
+        int ppipe = 1;
         if (get_tuned_choice(d->precision, d->m, d->n, d->k, &tv)) {
             part = tv.part;
             ranks = tv.ranks;
+            ppipe = tv.pipe;
         } else {
             for (const Prior& p : kPriors)
                 if (p.prec == d->precision && p.m == d->m && p.n == d->n && p.k == d->k) {
                     part = p.part;
                     ranks = p.ranks;
+                    ppipe = p.pipe;
                 }
         }
+        if (pipe == 0) pipe = ppipe;
+    }
+    if (pipe == 0) pipe = 1;
+    if (pipe == 2 && !pipe_allowed(d)) {
+        if (d->pipeline == 2) {
+            *st = KAMI_ENOTSUP;
+            return nullptr;
+        }
+        pipe = 1;
     }
     if (ranks == 0) ranks = default_ranks(d->precision, part, d->m, d->n, d->k);
     for (const auto& in : registry()) {
         if (in.prec == d->precision && in.m == d->m && in.n == d->n && in.k == d->k &&
-            in.part == part && in.ranks == ranks) {
+            in.part == part && in.ranks == ranks && in.pipe == pipe) {
             *st = KAMI_OK;
             return &in;
         }
     }
-    // AUTO falls back to any compiled instance of the shape (first registered)
+    // AUTO falls back to any compiled per-problem instance of the shape
     if (d->partition == KAMI_PART_AUTO) {
         for (const auto& in : registry())
-            if (in.prec == d->precision && in.m == d->m && in.n == d->n && in.k == d->k) {
+            if (in.prec == d->precision && in.m == d->m && in.n == d->n && in.k == d->k &&
+                in.pipe == 1) {
                 *st = KAMI_OK;
                 return &in;
             }
@@ -272,6 +295,8 @@ kami_status kami_block_gemm_query(const kami_gemm_desc* d, kami_gemm_config* out
     out->lds_bytes = in->lds;
     out->slices = 0;
     out->instance = (int32_t)(in - registry().data());
+    out->pipeline = in->pipe;
+    out->prefetch = in->prefetch;
     return KAMI_OK;
 }
 
@@ -338,13 +363,15 @@ kami_status kami_block_gemm_autotune(const kami_gemm_desc* d, const void* A, con
         return hip_fail(he);
     }
     float best_ms = 1e30f;
-    int best_part = 0, best_ranks = 0;
+    int best_part = 0, best_ranks = 0, best_pipe = 1;
     st = KAMI_ENOTSUP;
     for (const auto& in : registry()) {
         if (in.prec != d->precision || in.m != d->m || in.n != d->n || in.k != d->k) continue;
+        if (in.pipe == 2 && !pipe_allowed(d)) continue;
         kami_gemm_desc t = *d;
         t.partition = in.part;
         t.ranks = in.ranks;
+        t.pipeline = in.pipe;
         t.beta = 0.0;  // repeated runs must not feed back into C
         kami_status r = kami_block_gemm(&t, A, B, C, count, 0, 0, 0, stream);  // warm-up
         if (r != KAMI_OK) { st = r; break; }
@@ -359,16 +386,19 @@ kami_status kami_block_gemm_autotune(const kami_gemm_desc* d, const void* A, con
             best_ms = ms;
             best_part = in.part;
             best_ranks = in.ranks;
+            best_pipe = in.pipe;
         }
         st = KAMI_OK;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st != KAMI_OK) return st;
-    set_tuned_choice(d->precision, d->m, d->n, d->k, best_part, best_ranks);
+    set_tuned_choice(d->precision, d->m, d->n, d->k, best_part, best_ranks, best_pipe);
     if (chosen) {
         kami_gemm_desc t = *d;
         t.partition = KAMI_PART_AUTO;
+        t.ranks = 0;
+        t.pipeline = 0;
         return kami_block_gemm_query(&t, chosen);
     }
     return KAMI_OK;

@@ -17,7 +17,7 @@ kami_status device_check();
 const std::vector<BlockInstance>& registry();
 kami_status validate_desc(const kami_gemm_desc* d);
 const BlockInstance* resolve_block(const kami_gemm_desc* d, kami_status* st);
-void set_tuned_choice(int prec, int m, int n, int k, int part, int ranks);
+void set_tuned_choice(int prec, int m, int n, int k, int part, int ranks, int pipe);
 int elt_in(int prec);
 int elt_out(int prec);
 

@@ -7,7 +7,8 @@ For every compiled (shape, partition, P) instance of a precision, times two mode
                 TFLOPS = 2 M N K * count * iters / t (F64-1D:279-282).
 Each line is the reference's block-GEMM CSV schema
 `[hemeng_log],dim,M,N,K,NUM_RANK,NUM_ALLOC,TFLOPS,threads` (F64-1D:287) followed by
-mode, problems, iters, fraction of MFMA peak and max error vs the FP64 oracle.
+mode, problems, iters, fraction of MFMA peak, max error vs the FP64 oracle and the
+pipeline (1: one workgroup per problem, 2: persistent with deep prefetch).
 
     python -m kami_amd.sweep --prec f64 --sizes 64 128 --count 4096
 """
@@ -32,12 +33,13 @@ def instances(prec, shape):
     out = []
     for part in ("1d", "2d", "3d"):
         for r in (1, 2, 4, 8, 16):
-            try:
-                cfg = K.query(m, n, k, prec, partition=part, ranks=r)
-            except K.KamiError:
-                continue
-            if (part, r) not in [(o[0], o[1]) for o in out]:
-                out.append((part, r, cfg))
+            for pipe in (1, 2):
+                try:
+                    cfg = K.query(m, n, k, prec, partition=part, ranks=r, pipeline=pipe)
+                except K.KamiError:
+                    continue
+                if (part, r, pipe) not in [(o[0], o[1], o[3]) for o in out]:
+                    out.append((part, r, cfg, pipe))
     return out
 
 
@@ -90,12 +92,14 @@ def sweep(prec, shapes, count, target_tflop, steps, warmup, modes, out=sys.stdou
         # resident mode: at most 2048 workgroups, repeats sized to ~target_tflop per call
         cres = min(count, 2048)
         it_res = max(1, int(round(target_tflop * 1e12 / (2.0 * m * n * k * cres))))
-        for (part, r, cfg) in instances(prec, shape):
+        for (part, r, cfg, pipe) in instances(prec, shape):
             for mode in modes:
+                if mode == "resident" and pipe == 2:
+                    continue  # the persistent form runs iters = 1 only
                 it = it_res if mode == "resident" else 1
                 cnt = cres if mode == "resident" else count
                 fn = lambda: K.block_gemm(A[:cnt], B[:cnt], C[:cnt], precision=prec,  # noqa
-                                          partition=part, ranks=r, iters=it)
+                                          partition=part, ranks=r, iters=it, pipeline=pipe)
                 ms = time_ms(fn, steps, warmup)
                 tflops = 2.0 * m * n * k * cnt * it / (ms * 1e-3) / 1e12
                 err = max_err(prec, A, B, C) if it == 1 else float("nan")
@@ -103,10 +107,10 @@ def sweep(prec, shapes, count, target_tflop, steps, warmup, modes, out=sys.stdou
                 dim = {1: "1d", 2: "2d", 3: "3d"}[cfg["partition"]]
                 waves = cfg["waves_m"] * cfg["waves_n"] * cfg["waves_k"]
                 print(f"[hemeng_log],{dim},{m},{n},{k},{r},{waves},{tflops:.3f},{cfg['threads']},"
-                      f"{mode},{cnt},{it},{frac:.4f},{err:.3e}", file=out, flush=True)
+                      f"{mode},{cnt},{it},{frac:.4f},{err:.3e},pipeline{pipe}", file=out, flush=True)
                 key = (shape, mode)
                 if key not in best or tflops > best[key][2]:
-                    best[key] = (part, r, tflops)
+                    best[key] = (part, r, tflops, pipe)
     return best
 
 
@@ -126,8 +130,9 @@ def main(argv=None):
     prec = PRECS[args.prec]
     shapes = [(s, s, s) for s in args.sizes] + [(256, 256, kk) for kk in args.lowrank]
     best = sweep(prec, shapes, args.count, args.target_tflop, args.steps, args.warmup, args.modes)
-    summary = {f"{s[0]}x{s[1]}x{s[2]}/{mode}": {"partition": p, "ranks": r, "tflops": t}
-               for (s, mode), (p, r, t) in best.items()}
+    summary = {f"{s[0]}x{s[1]}x{s[2]}/{mode}": {"partition": p, "ranks": r, "tflops": t,
+                                                "pipeline": pp}
+               for (s, mode), (p, r, t, pp) in best.items()}
     print("[best] " + json.dumps(summary), flush=True)
 
 

@@ -24,7 +24,7 @@ def test_status_strings_and_version():
 
 def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_abi.GemmDesc) == 80
-    assert ctypes.sizeof(_abi.GemmConfig) == 40
+    assert ctypes.sizeof(_abi.GemmConfig) == 48
     assert ctypes.sizeof(_abi.Problem) == 64
     assert ctypes.sizeof(_abi.BatchedOpts) == 32
     assert ctypes.sizeof(_abi.Bsr) == 40

@@ -224,3 +224,34 @@ def test_half_normal_inputs_tolerance(bf16, shape, part, dev):
     err = np.abs(C - ref)
     assert np.all(err[:2] <= 1e-2 * mag)          # the stated tolerance, relative to |A||B|
     assert np.max(err) / np.max(np.abs(ref)) < 1e-5  # FP32 accumulation in practice
+
+
+PIPE_CASES = [(prec, s, r) for prec, sizes in ((K.PREC_F64, (32, 64, 128)), (K.PREC_F16_F32, (64, 128)),
+                                               (K.PREC_BF16_F32, (64, 128)))
+              for s in sizes for r in (2, 4)]
+
+
+@pytest.mark.parametrize("prec,s,r", PIPE_CASES)
+def test_persistent_pipeline_matches_oracle(prec, s, r, dev):
+    """The persistent 2D form walks several problems per workgroup and prefetches across
+    problem boundaries: counts below, equal to and far above the grid size."""
+    try:
+        K.query(s, s, s, prec, partition="2d", ranks=r, pipeline=2)
+    except K.KamiError:
+        pytest.skip("no persistent instance")
+    rng = np.random.default_rng(s + r)
+    for count in (1, 7, 2500):
+        A = rng.integers(-3, 4, size=(count, s, s)).astype(np.float64)
+        B = rng.integers(-3, 4, size=(count, s, s)).astype(np.float64)
+        C0 = rng.integers(-3, 4, size=(count, s, s)).astype(np.float64)
+        if prec == K.PREC_F64:
+            Ad, Bd = to_dev(A, dev), to_dev(B, dev)
+            Cd = to_dev(C0, dev)
+        else:
+            tdt = torch.float16 if prec == K.PREC_F16_F32 else torch.bfloat16
+            Ad, Bd = to_dev(A, dev).to(tdt), to_dev(B, dev).to(tdt)
+            Cd = to_dev(C0, dev).float()
+        K.block_gemm(Ad, Bd, Cd, precision=prec, partition="2d", ranks=r, pipeline=2,
+                     alpha=2.0, beta=-1.0)
+        ref = 2.0 * np.einsum("pik,pjk->pij", A, B) - C0
+        np.testing.assert_array_equal(Cd.double().cpu().numpy(), ref)
