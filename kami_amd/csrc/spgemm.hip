@@ -13,10 +13,11 @@
 //   fill   -- the bitmap is rebuilt and its set bits are written in ascending order,
 //             each thread placing the bits of one 32-bit word after the prefix popcount
 //             of the words before it;
-//   numeric -- one wave per output block (i, j): for each block (i, k) of A the block
-//             (k, j) of B is found by binary search in B's sorted row k; A fragments are
-//             loaded straight into registers, B blocks (row-major K x N) are transposed
-//             through the wave's LDS into the K-contiguous MFMA layout.
+//   numeric -- one wave per output block (i, j): the row i comes from a 64-way search
+//             of rowptrC; the lanes binary-search B's sorted rows k of all blocks (i, k)
+//             of A at once for column j; A fragments are loaded straight into
+//             registers, B blocks (row-major K x N) are transposed through the wave's LDS
+//             into the K-contiguous MFMA layout.
 // All integer phases are exact and deterministic.  B may have at most 491520 block
 // columns (a 60 KiB bitmap).
 #include <algorithm>
@@ -188,11 +189,17 @@ spgemm_numeric_kernel(int mb, int64_t nnzc, const int32_t* rpA, const int32_t* c
     const acc_t alpha = (acc_t)alpha_d;
 
     for (int64_t cb = blockIdx.x; cb < nnzc; cb += gridDim.x) {
-        // block row of output block cb: last row with rpC[row] <= cb
+        // block row of output block cb: the last row with rpC[row] <= cb, found by a
+        // 64-way search (each round the lanes probe 64 evenly spaced row pointers)
         int lo = 0, hi = mb;
         while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if ((int64_t)rpC[mid] <= cb) lo = mid; else hi = mid;
+            const int step = (hi - lo + 63) / 64;
+            const int idx = lo + (lane + 1) * step;
+            const bool le = idx < hi && (int64_t)rpC[idx] <= cb;
+            const int c = __popcll(__ballot(le));  // probes are monotone: a prefix is set
+            const int nlo = lo + c * step;
+            hi = min(hi, nlo + step);
+            lo = nlo;
         }
         const int i = lo, j = ciC[cb];
         accv_t acc[G::IM][G::IM];
@@ -203,10 +210,22 @@ spgemm_numeric_kernel(int mb, int64_t nnzc, const int32_t* rpA, const int32_t* c
 #pragma unroll
                 for (int v = 0; v < 4; ++v) acc[a][b][v] = 0;
 
-        for (int t = rpA[i]; t < rpA[i + 1]; ++t) {
-            const int k = ciA[t];
-            const int l = find_col(ciB, rpB[k], rpB[k + 1], j);
-            if (l < 0) continue;  // wave-uniform
+        // every lane searches the B row of one block of A's row i for column j; the
+        // matches are then multiplied in ascending order of A's blocks
+        const int a0 = rpA[i], a1 = rpA[i + 1];
+        for (int base = a0; base < a1; base += 64) {
+          const int tl = base + lane;
+          int lmatch = -1;
+          if (tl < a1) {
+              const int k = ciA[tl];
+              lmatch = find_col(ciB, rpB[k], rpB[k + 1], j);
+          }
+          uint64_t todo = __ballot(lmatch >= 0);
+          while (todo) {
+            const int src = __ffsll((unsigned long long)todo) - 1;
+            todo &= todo - 1;
+            const int t = base + src;
+            const int l = __shfl(lmatch, src);
             const in_t* Ablk = vA + (int64_t)t * BS * BS;
             const in_t* Bblk = vB + (int64_t)l * BS * BS;
             // B block (row-major K x N) -> LDS as N rows of K (transposed)
@@ -239,6 +258,7 @@ spgemm_numeric_kernel(int mb, int64_t nnzc, const int32_t* rpA, const int32_t* c
                 }
             }
             __syncthreads();
+          }
         }
         out_t* Cblk = vC + cb * BS * BS;
 #pragma unroll
