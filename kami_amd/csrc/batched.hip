@@ -29,166 +29,34 @@ struct WorkItem {
     int32_t prob, row0, col0, variant;
 };
 
-// Tile geometry: TM x TN output tile, WM x WN waves (4 waves), K staged KC at a time.
-template <class AT, int TM_, int TN_, int WM_, int WN_, int KC_>
-struct TileGeom {
-    static constexpr int TM = TM_, TN = TN_, WM = WM_, WN = WN_, KC = KC_;
+// Staging shared by every tile variant: a stage holds up to TMAX rows of A and TNMAX rows
+// of B (or KC rows of TNMAX columns when B is K x N), KC elements of K each, in LDS rows
+// padded by 16 bytes.  Pieces are 16 bytes; a thread owns pieces tid + i * NT.
+template <class AT>
+struct StreamGeom {
     static constexpr int NT = 256;
-    static_assert(WM * WN * 64 == NT, "four waves");
-    static constexpr int SUBM = TM / WM, SUBN = TN / WN, IM = SUBM / 16, JN = SUBN / 16;
     static constexpr int ELT = (int)sizeof(typename AT::in_t), VE = 16 / ELT;
+    static constexpr int KC = ELT == 8 ? 16 : 64;
+    static constexpr int TMAX = 128, TNMAX = 64;
     static constexpr int ROWB = KC * ELT + 16;
-    static constexpr int A_LDS = TM * ROWB, B_LDS = TN * ROWB, STAGE = A_LDS + B_LDS;
+    static constexpr int A_REGION = TMAX * ROWB;
+    static constexpr int STAGE = (TMAX + TNMAX) * ROWB;
     static constexpr int LDS = 2 * STAGE;
-    static constexpr int CA = TM * KC / VE, CB = TN * KC / VE;
-    static constexpr int NA = (CA + NT - 1) / NT, NB = (CB + NT - 1) / NT;
-    static_assert(SUBM % 16 == 0 && SUBN % 16 == 0 && KC % AT::KSTEP == 0, "tile");
+    static constexpr int CPR = KC / VE;  // pieces per staged row
+    static constexpr int NP = ((TMAX + TNMAX) * CPR + NT - 1) / NT;
+    static_assert((KC * TNMAX) / VE <= TNMAX * CPR, "K x N staging fits the B region");
 };
 
-template <class AT, class TG>
-__device__ void tile_gemm(const kami_problem& P, int row0, int col0, unsigned char* smem,
-                          double alpha_d, double beta_d, bool b_kn) {
-    using in_t = typename AT::in_t;
-    using acc_t = typename AT::acc_t;
-    using out_t = typename AT::out_t;
-    using frag_t = typename AT::frag_t;
-    using accv_t = typename AT::accv_t;
-    constexpr int FB = AT::FRAG_BYTES;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave / TG::WN, wn = wave % TG::WN;
-    const int M = P.m, N = P.n, K = P.k;
-    const int64_t lda = P.lda, ldb = P.ldb, ldc = P.ldc;
-    const in_t* A = reinterpret_cast<const in_t*>(P.A);
-    const in_t* B = reinterpret_cast<const in_t*>(P.B);
-    out_t* C = reinterpret_cast<out_t*>(P.C);
-    const int mlim = M - row0, nlim = N - col0;  // rows / cols of the tile inside C
-
-    accv_t acc[TG::IM][TG::JN];
-#pragma unroll
-    for (int i = 0; i < TG::IM; ++i)
-#pragma unroll
-        for (int j = 0; j < TG::JN; ++j)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[i][j][v] = 0;
-
-    u4_t stA[TG::NA], stB[TG::NB];
-    auto load_stage = [&](int k0) {
-        constexpr int CPR = TG::KC / TG::VE;
-#pragma unroll
-        for (int i = 0; i < TG::NA; ++i) {
-            const int c = tid + i * TG::NT;
-            const int r = c / CPR, kc = (c % CPR) * TG::VE;
-            u4_t v = {0, 0, 0, 0};
-            if (c < TG::CA && r < mlim && k0 + kc < K)
-                v = load_raw<16>(A + (int64_t)(row0 + r) * lda + k0 + kc);
-            stA[i] = v;
-        }
-        if (!b_kn) {
-#pragma unroll
-            for (int i = 0; i < TG::NB; ++i) {
-                const int c = tid + i * TG::NT;
-                const int r = c / CPR, kc = (c % CPR) * TG::VE;
-                u4_t v = {0, 0, 0, 0};
-                if (c < TG::CB && r < nlim && k0 + kc < K)
-                    v = load_raw<16>(B + (int64_t)(col0 + r) * ldb + k0 + kc);
-                stB[i] = v;
-            }
-        } else {
-            constexpr int CPRN = TG::TN / TG::VE;
-#pragma unroll
-            for (int i = 0; i < TG::NB; ++i) {
-                const int c = tid + i * TG::NT;
-                const int kr = c / CPRN, jc = (c % CPRN) * TG::VE;
-                u4_t v = {0, 0, 0, 0};
-                if (c < TG::CB && jc < nlim && k0 + kr < K)
-                    v = load_raw<16>(B + (int64_t)(k0 + kr) * ldb + col0 + jc);
-                stB[i] = v;
-            }
-        }
-    };
-    auto write_stage = [&](int buf) {
-        unsigned char* base = smem + buf * TG::STAGE;
-        constexpr int CPR = TG::KC / TG::VE;
-#pragma unroll
-        for (int i = 0; i < TG::NA; ++i) {
-            const int c = tid + i * TG::NT;
-            if (c < TG::CA) store_raw<16>(base + (c / CPR) * TG::ROWB + (c % CPR) * 16, stA[i]);
-        }
-        unsigned char* bb = base + TG::A_LDS;
-        if (!b_kn) {
-#pragma unroll
-            for (int i = 0; i < TG::NB; ++i) {
-                const int c = tid + i * TG::NT;
-                if (c < TG::CB) store_raw<16>(bb + (c / CPR) * TG::ROWB + (c % CPR) * 16, stB[i]);
-            }
-        } else {
-            constexpr int CPRN = TG::TN / TG::VE;
-#pragma unroll
-            for (int i = 0; i < TG::NB; ++i) {
-                const int c = tid + i * TG::NT;
-                if (c < TG::CB) {
-                    const int kr = c / CPRN, jc = (c % CPRN) * TG::VE;
-                    const in_t* e = reinterpret_cast<const in_t*>(&stB[i]);
-#pragma unroll
-                    for (int q = 0; q < TG::VE; ++q)
-                        *reinterpret_cast<in_t*>(bb + (jc + q) * TG::ROWB + kr * TG::ELT) = e[q];
-                }
-            }
-        }
-    };
-    auto compute = [&](int buf) {
-        const unsigned char* base = smem + buf * TG::STAGE;
-        const unsigned char* bb = base + TG::A_LDS;
-#pragma unroll
-        for (int ks = 0; ks < TG::KC / AT::KSTEP; ++ks) {
-            const int kk = (ks * AT::KSTEP + AT::KV * (lane >> 4)) * TG::ELT;
-            frag_t af[TG::IM];
-#pragma unroll
-            for (int im = 0; im < TG::IM; ++im)
-                af[im] = bitcast<frag_t>(
-                    load_raw<FB>(base + (wm * TG::SUBM + im * 16 + (lane & 15)) * TG::ROWB + kk));
-#pragma unroll
-            for (int jn = 0; jn < TG::JN; ++jn) {
-                if (wn * TG::SUBN + jn * 16 >= nlim) continue;  // atom outside C: skip
-                const frag_t bf = bitcast<frag_t>(
-                    load_raw<FB>(bb + (wn * TG::SUBN + jn * 16 + (lane & 15)) * TG::ROWB + kk));
-#pragma unroll
-                for (int im = 0; im < TG::IM; ++im)
-                    if (wm * TG::SUBM + im * 16 < mlim) AT::mma(acc[im][jn], af[im], bf);
-            }
-        }
-    };
-
-    const int nch = (K + TG::KC - 1) / TG::KC;
-    load_stage(0);
-    write_stage(0);
-    __syncthreads();
-    for (int c = 0; c < nch; ++c) {
-        const bool has_next = c + 1 < nch;
-        if (has_next) load_stage((c + 1) * TG::KC);
-        compute(c & 1);
-        if (has_next) write_stage((c & 1) ^ 1);
-        __syncthreads();
-    }
-
-    const acc_t alpha = (acc_t)alpha_d, beta = (acc_t)beta_d;
-#pragma unroll
-    for (int im = 0; im < TG::IM; ++im)
-#pragma unroll
-        for (int jn = 0; jn < TG::JN; ++jn) {
-            const int rb = wm * TG::SUBM + im * 16, cb = wn * TG::SUBN + jn * 16;
-            if (rb >= mlim || cb >= nlim) continue;
-            const int col = col0 + cb + (lane & 15);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = row0 + rb + AT::out_row(lane, v);
-                out_t* dst = C + (int64_t)row * ldc + col;
-                acc_t val = alpha * acc[im][jn][v];
-                if (beta != (acc_t)0) val += beta * (acc_t)(*dst);
-                *dst = (out_t)val;
-            }
-        }
-}
+// Tile geometry of a variant: TM x TN output tile, WM x WN waves (4 waves).
+template <class AT, int TM_, int TN_, int WM_, int WN_>
+struct TileGeom {
+    using SG = StreamGeom<AT>;
+    static constexpr int TM = TM_, TN = TN_, WM = WM_, WN = WN_;
+    static_assert(WM * WN * 64 == SG::NT, "four waves");
+    static_assert(TM <= SG::TMAX && TN <= SG::TNMAX, "tile fits the stage");
+    static constexpr int SUBM = TM / WM, SUBN = TN / WN, IM = SUBM / 16, JN = SUBN / 16;
+    static_assert(SUBM % 16 == 0 && SUBN % 16 == 0 && SG::KC % AT::KSTEP == 0, "tile");
+};
 
 // Tile variants.  Index = variant id stored in the work items.
 //   0: 64 x 64 tile, 2 x 2 waves (2D)       1: 32 x 32 tile, 2 x 2 waves (2D)
@@ -199,13 +67,14 @@ static const int kVariantTN[kVariants] = {64, 32, 64, 64};
 
 template <class AT>
 struct Variants {
-    static constexpr int KC = (sizeof(typename AT::in_t) == 8) ? 16 : 64;
-    using V0 = TileGeom<AT, 64, 64, 2, 2, KC>;
-    using V1 = TileGeom<AT, 32, 32, 2, 2, KC>;
-    using V2 = TileGeom<AT, 64, 64, 4, 1, KC>;
-    using V3 = TileGeom<AT, 128, 64, 2, 2, KC>;
-    static constexpr int LDS = cmax(cmax(V0::LDS, V1::LDS), cmax(V2::LDS, V3::LDS));
+    using V0 = TileGeom<AT, 64, 64, 2, 2>;
+    using V1 = TileGeom<AT, 32, 32, 2, 2>;
+    using V2 = TileGeom<AT, 64, 64, 4, 1>;
+    using V3 = TileGeom<AT, 128, 64, 2, 2>;
 };
+
+__device__ inline int variant_tm(int v) { return v == 1 ? 32 : (v == 3 ? 128 : 64); }
+__device__ inline int variant_tn(int v) { return v == 1 ? 32 : 64; }
 
 struct BatchArgs {
     const kami_problem* probs;
@@ -216,27 +85,260 @@ struct BatchArgs {
     int32_t b_kn;
 };
 
+// A queue entry as the stream sees it (identical in every thread of a workgroup).
+struct TileRef {
+    int prob, row0, col0, variant, nch;
+    bool valid;
+};
+
+// Loads the pieces of stage (tile t, K chunk c) into registers; rows outside C and K
+// beyond the problem are zero.
 template <class AT>
-__global__ void __launch_bounds__(256) batched_persistent_kernel(BatchArgs args) {
+__device__ __forceinline__ void stream_load(const BatchArgs& a, const TileRef& t, int c, bool b_kn,
+                            u4_t (&r)[StreamGeom<AT>::NP]) {
+    using SG = StreamGeom<AT>;
+    using in_t = typename AT::in_t;
+    const kami_problem P = a.probs[t.prob];
+    const int tm = variant_tm(t.variant), tn = variant_tn(t.variant);
+    const int mlim = P.m - t.row0, nlim = P.n - t.col0, K = P.k, k0 = c * SG::KC;
+    const in_t* A = reinterpret_cast<const in_t*>(P.A);
+    const in_t* B = reinterpret_cast<const in_t*>(P.B);
+    const int ca = tm * SG::CPR;
+#pragma unroll
+    for (int i = 0; i < SG::NP; ++i) {
+        const int idx = threadIdx.x + i * SG::NT;
+        u4_t v = {0, 0, 0, 0};
+        if (idx < ca) {
+            const int row = idx / SG::CPR, kc = (idx % SG::CPR) * SG::VE;
+            if (row < mlim && k0 + kc < K) v = load_raw<16>(A + (int64_t)(t.row0 + row) * P.lda + k0 + kc);
+        } else if (!b_kn) {
+            const int j = idx - ca;
+            const int row = j / SG::CPR, kc = (j % SG::CPR) * SG::VE;
+            if (row < tn && row < nlim && k0 + kc < K)
+                v = load_raw<16>(B + (int64_t)(t.col0 + row) * P.ldb + k0 + kc);
+        } else {
+            const int j = idx - ca, ppr = tn / SG::VE;  // pieces per K row of the tile
+            const int kr = j / ppr, jc = (j % ppr) * SG::VE;
+            if (kr < SG::KC && jc < nlim && k0 + kr < K)
+                v = load_raw<16>(B + (int64_t)(k0 + kr) * P.ldb + t.col0 + jc);
+        }
+        r[i] = v;
+    }
+}
+
+template <class AT>
+__device__ __forceinline__ void stream_write(int variant, bool b_kn, const u4_t (&r)[StreamGeom<AT>::NP],
+                             unsigned char* buf) {
+    using SG = StreamGeom<AT>;
+    using in_t = typename AT::in_t;
+    const int tm = variant_tm(variant), tn = variant_tn(variant);
+    const int ca = tm * SG::CPR;
+#pragma unroll
+    for (int i = 0; i < SG::NP; ++i) {
+        const int idx = threadIdx.x + i * SG::NT;
+        if (idx < ca) {
+            store_raw<16>(buf + (idx / SG::CPR) * SG::ROWB + (idx % SG::CPR) * 16, r[i]);
+        } else if (!b_kn) {
+            const int j = idx - ca;
+            if (j < tn * SG::CPR)
+                store_raw<16>(buf + SG::A_REGION + (j / SG::CPR) * SG::ROWB + (j % SG::CPR) * 16, r[i]);
+        } else {
+            const int j = idx - ca, ppr = tn / SG::VE;
+            const int kr = j / ppr, jc = (j % ppr) * SG::VE;
+            if (kr < SG::KC) {
+                const in_t* e = reinterpret_cast<const in_t*>(&r[i]);
+#pragma unroll
+                for (int q = 0; q < SG::VE; ++q)
+                    *reinterpret_cast<in_t*>(buf + SG::A_REGION + (jc + q) * SG::ROWB + kr * SG::ELT) = e[q];
+            }
+        }
+    }
+}
+
+// Stream state of a workgroup: the current tile and two queued behind it, the position
+// of the next stage to load, and a two-deep register ring of staged pieces (slot 0 =
+// the stage after the one in LDS, slot 1 = the stage after that).
+template <class AT>
+struct Stream {
+    TileRef T[3];
+    int lt, lc;  // load cursor: tile index into T, K chunk
+    u4_t ring0[StreamGeom<AT>::NP], ring1[StreamGeom<AT>::NP];
+    int v0, v1;  // variant of the stage in each slot, -1 = empty
+    int buf;     // LDS buffer holding the stage being computed
+};
+
+__device__ __forceinline__ TileRef fetch_tile(const BatchArgs& a, unsigned int* s_item) {
+    if (threadIdx.x == 0) *s_item = atomicAdd(a.counter, 1u);
+    __syncthreads();
+    const int64_t it = *s_item;
+    __syncthreads();
+    TileRef t;
+    t.valid = it < a.n_items;  // identical in every thread: the exit is uniform
+    if (t.valid) {
+        const WorkItem w = a.items[it];
+        t.prob = w.prob;
+        t.row0 = w.row0;
+        t.col0 = w.col0;
+        t.variant = w.variant;
+        t.nch = 0;
+    } else {
+        t.prob = t.row0 = t.col0 = t.variant = t.nch = 0;
+    }
+    return t;
+}
+
+template <class AT>
+__device__ __forceinline__ void set_chunks(const BatchArgs& a, TileRef& t) {
+    if (t.valid) t.nch = (a.probs[t.prob].k + StreamGeom<AT>::KC - 1) / StreamGeom<AT>::KC;
+}
+
+// Field-wise selection: a runtime index into the private array, or a select between
+// their addresses, would keep the whole stream state in scratch memory.
+__device__ __forceinline__ TileRef pick_tile(int i, const TileRef& x, const TileRef& y,
+                                             const TileRef& z) {
+    TileRef t;
+    t.prob = i == 0 ? x.prob : (i == 1 ? y.prob : z.prob);
+    t.row0 = i == 0 ? x.row0 : (i == 1 ? y.row0 : z.row0);
+    t.col0 = i == 0 ? x.col0 : (i == 1 ? y.col0 : z.col0);
+    t.variant = i == 0 ? x.variant : (i == 1 ? y.variant : z.variant);
+    t.nch = i == 0 ? x.nch : (i == 1 ? y.nch : z.nch);
+    t.valid = i == 0 ? x.valid : (i == 1 ? y.valid : z.valid);
+    return t;
+}
+
+// Load the stage at the cursor into `dst` (if any) and advance the cursor.
+template <class AT>
+__device__ __forceinline__ int load_at_cursor(const BatchArgs& a, Stream<AT>& st, bool b_kn,
+                              u4_t (&dst)[StreamGeom<AT>::NP]) {
+    if (st.lt > 2) return -1;
+    const TileRef t = pick_tile(st.lt, st.T[0], st.T[1], st.T[2]);
+    if (!t.valid) return -1;
+    stream_load<AT>(a, t, st.lc, b_kn, dst);
+    const int v = t.variant;
+    if (++st.lc >= t.nch) {
+        st.lc = 0;
+        ++st.lt;
+    }
+    return v;
+}
+
+template <class AT, class V>
+__device__ __forceinline__ void run_tile(const BatchArgs& a, Stream<AT>& st, unsigned char* smem, bool b_kn) {
+    using SG = StreamGeom<AT>;
+    using acc_t = typename AT::acc_t;
+    using out_t = typename AT::out_t;
+    using frag_t = typename AT::frag_t;
+    using accv_t = typename AT::accv_t;
+    constexpr int FB = AT::FRAG_BYTES;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave / V::WN, wn = wave % V::WN;
+    const TileRef cur = st.T[0];
+    const kami_problem P = a.probs[cur.prob];
+    const int mlim = P.m - cur.row0, nlim = P.n - cur.col0;
+
+    accv_t acc[V::IM][V::JN];
+#pragma unroll
+    for (int i = 0; i < V::IM; ++i)
+#pragma unroll
+        for (int j = 0; j < V::JN; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[i][j][v] = 0;
+
+    for (int c = 0; c < cur.nch; ++c) {
+        const unsigned char* base = smem + st.buf * SG::STAGE;
+        const unsigned char* bb = base + SG::A_REGION;
+#pragma unroll
+        for (int ks = 0; ks < SG::KC / AT::KSTEP; ++ks) {
+            const int kk = (ks * AT::KSTEP + AT::KV * (lane >> 4)) * SG::ELT;
+            frag_t af[V::IM];
+#pragma unroll
+            for (int im = 0; im < V::IM; ++im)
+                af[im] = bitcast<frag_t>(
+                    load_raw<FB>(base + (wm * V::SUBM + im * 16 + (lane & 15)) * SG::ROWB + kk));
+#pragma unroll
+            for (int jn = 0; jn < V::JN; ++jn) {
+                if (wn * V::SUBN + jn * 16 >= nlim) continue;  // atom outside C: skip
+                const frag_t bf = bitcast<frag_t>(
+                    load_raw<FB>(bb + (wn * V::SUBN + jn * 16 + (lane & 15)) * SG::ROWB + kk));
+#pragma unroll
+                for (int im = 0; im < V::IM; ++im)
+                    if (wm * V::SUBM + im * 16 < mlim) AT::mma(acc[im][jn], af[im], bf);
+            }
+        }
+        // the stage after this one goes to the other buffer; the ring shifts and the
+        // stage two ahead (possibly of a later tile) is loaded
+        if (st.v0 >= 0) stream_write<AT>(st.v0, b_kn, st.ring0, smem + (st.buf ^ 1) * SG::STAGE);
+#pragma unroll
+        for (int i = 0; i < SG::NP; ++i) st.ring0[i] = st.ring1[i];
+        st.v0 = st.v1;
+        st.v1 = load_at_cursor<AT>(a, st, b_kn, st.ring1);
+        __syncthreads();
+        st.buf ^= 1;
+    }
+
+    const acc_t alpha = (acc_t)a.alpha, beta = (acc_t)a.beta;
+    out_t* C = reinterpret_cast<out_t*>(P.C);
+#pragma unroll
+    for (int im = 0; im < V::IM; ++im)
+#pragma unroll
+        for (int jn = 0; jn < V::JN; ++jn) {
+            const int rb = wm * V::SUBM + im * 16, cb = wn * V::SUBN + jn * 16;
+            if (rb >= mlim || cb >= nlim) continue;
+            const int col = cur.col0 + cb + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = cur.row0 + rb + AT::out_row(lane, v);
+                out_t* dst = C + (int64_t)row * P.ldc + col;
+                acc_t val = alpha * acc[im][jn][v];
+                if (beta != (acc_t)0) val += beta * (acc_t)(*dst);
+                *dst = (out_t)val;
+            }
+        }
+}
+
+template <class AT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) batched_persistent_kernel(BatchArgs args) {
+    using SG = StreamGeom<AT>;
     using VS = Variants<AT>;
-    __shared__ __attribute__((aligned(16))) unsigned char smem[VS::LDS];
+    __shared__ __attribute__((aligned(16))) unsigned char smem[SG::LDS];
     __shared__ unsigned int s_item;
     const bool b_kn = args.b_kn != 0;
-    for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(args.counter, 1u);
-        __syncthreads();
-        const int64_t it = s_item;
-        __syncthreads();
-        if (it >= args.n_items) break;  // every wave of the workgroup sees the same value
-        const WorkItem w = args.items[it];
-        const kami_problem P = args.probs[w.prob];
-        switch (w.variant) {
-            case 0: tile_gemm<AT, typename VS::V0>(P, w.row0, w.col0, smem, args.alpha, args.beta, b_kn); break;
-            case 1: tile_gemm<AT, typename VS::V1>(P, w.row0, w.col0, smem, args.alpha, args.beta, b_kn); break;
-            case 2: tile_gemm<AT, typename VS::V2>(P, w.row0, w.col0, smem, args.alpha, args.beta, b_kn); break;
-            default: tile_gemm<AT, typename VS::V3>(P, w.row0, w.col0, smem, args.alpha, args.beta, b_kn); break;
+    Stream<AT> st;
+    st.T[0] = fetch_tile(args, &s_item);
+    if (!st.T[0].valid) return;  // uniform
+    st.T[1] = fetch_tile(args, &s_item);
+    st.T[2] = fetch_tile(args, &s_item);
+    set_chunks<AT>(args, st.T[0]);
+    set_chunks<AT>(args, st.T[1]);
+    set_chunks<AT>(args, st.T[2]);
+    st.lt = 0;
+    st.lc = 0;
+    st.buf = 0;
+    // prologue: first stage into LDS buffer 0, the next two into the ring
+    const int vfirst = load_at_cursor<AT>(args, st, b_kn, st.ring0);
+    stream_write<AT>(vfirst, b_kn, st.ring0, smem);
+    st.v0 = load_at_cursor<AT>(args, st, b_kn, st.ring0);
+    st.v1 = load_at_cursor<AT>(args, st, b_kn, st.ring1);
+    __syncthreads();
+    while (st.T[0].valid) {
+        switch (st.T[0].variant) {
+            case 0: run_tile<AT, typename VS::V0>(args, st, smem, b_kn); break;
+            case 1: run_tile<AT, typename VS::V1>(args, st, smem, b_kn); break;
+            case 2: run_tile<AT, typename VS::V2>(args, st, smem, b_kn); break;
+            default: run_tile<AT, typename VS::V3>(args, st, smem, b_kn); break;
         }
-        __syncthreads();
+        // the next queued tile becomes current; one more is popped behind it
+        st.T[0] = st.T[1];
+        st.T[1] = st.T[2];
+        st.T[2] = fetch_tile(args, &s_item);
+        set_chunks<AT>(args, st.T[2]);
+        st.lt -= 1;
+        // the cursor may have stopped at the end of the queue; if a tile appeared, the
+        // ring is refilled from it
+        if (st.v1 < 0 && st.lt >= 0) {
+            if (st.v0 < 0) st.v0 = load_at_cursor<AT>(args, st, b_kn, st.ring0);
+            if (st.v0 >= 0) st.v1 = load_at_cursor<AT>(args, st, b_kn, st.ring1);
+        }
     }
 }
 
