@@ -92,7 +92,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--path", choices=["auto", "native", "split"], default="auto")
+    ap.add_argument("--path", choices=["auto", "native", "split", "split_fullk"], default="auto")
     ap.add_argument("--m", type=int, default=128)
     ap.add_argument("--problems", type=int, default=4096)
     ap.add_argument("--err-sample", type=int, default=64)
@@ -123,15 +123,19 @@ def main():
 
     results = {}
     outs = {}
-    paths = [("native", K.PREC_F64), ("split", K.PREC_F64_SPLIT_F16)]
-    for name, prec in paths:
+    # split: persistent form with per-chunk exponents (the default); split_fullk: one
+    # workgroup per problem with exponents over the full K, reported for comparison
+    paths = [("native", K.PREC_F64, 0), ("split", K.PREC_F64_SPLIT_F16, 0),
+             ("split_fullk", K.PREC_F64_SPLIT_F16, 1)]
+    for name, prec, pipe in paths:
         try:
-            cfg = K.query(m, n, k, prec)
+            cfg = K.query(m, n, k, prec, pipeline=pipe)
         except K.KamiError as e:
             results[name] = {"available": False, "reason": str(e)}
             continue
         Cd = torch.empty((count, m, n), dtype=torch.float64, device=dev)
-        fn = lambda: K.block_gemm(Ad, Bd, Cd, precision=prec, stream=stream)  # noqa: E731
+        fn = lambda: K.block_gemm(Ad, Bd, Cd, precision=prec, pipeline=pipe,  # noqa: E731
+                                  stream=stream)
         ms = time_calls(fn, args.steps, args.warmup, stream)
         C = Cd.cpu().numpy()
         outs[name] = C
@@ -144,7 +148,7 @@ def main():
         print(f"[bench] {name}: {ms:.4f} ms/step  {tflops:.2f} TFLOPS  err={err}",
               file=sys.stderr, flush=True)
 
-    ok = [p for p in ("split", "native") if results.get(p, {}).get("available")
+    ok = [p for p in ("split", "split_fullk", "native") if results.get(p, {}).get("available")
           and (p == "native" or results[p]["max_rel_err"]["normwise"] <= 1e-13)]
     if args.path == "auto":
         chosen = max(ok, key=lambda p: results[p]["tflops"])

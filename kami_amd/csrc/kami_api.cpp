@@ -99,10 +99,8 @@ static int default_ranks(int prec, int part, int m, int n, int k) {
     return 2;
 }
 
-// Partition chosen for KAMI_PART_AUTO before any tuning data exists: 1D, the
-// reference's best FP64 and FP16 partition at these sizes
-// (logs/block_gemm/square/H200/fp64_block_square_1d_H200.csv:4,
-//  logs/block_gemm/square/H200/fp16_block_square_1d_H200.csv:4,8).
+// Choices recorded by kami_block_gemm_autotune per (precision, M, N, K); AUTO uses them
+// ahead of the measured priors.
 static std::mutex g_tune_mu;
 struct TuneKey {
     int prec, m, n, k;

@@ -294,9 +294,195 @@ split_gemm_kernel(SplitArgs args) {
         }
 }
 
+// Persistent form.  A workgroup walks problems blockIdx.x, + gridDim.x, ... and reads
+// every operand once: the exponents are taken per row and per 32-wide K chunk (so no
+// separate exponent pass over K is needed), the chunk is sliced from registers, and the
+// next chunk -- chunk 0 of the next problem after the last one -- is loaded while the
+// MFMAs of the current chunk run.  A chunk's level sums are combined in FP64 with the
+// weights 2^(-BETA g) and then scaled by 2^(e_i + f_j) of that chunk into the
+// accumulator, which holds C in absolute units.  Per-chunk exponents bound each chunk's
+// digits by the chunk's own row maximum, so the truncation error is no larger than
+// with exponents over the full K.
+template <class SAT, class G>
+__global__ void __launch_bounds__(G::NT)
+split_pipe_kernel(SplitArgs args, int64_t count) {
+    using frag_t = typename SAT::frag_t;
+    using accv_t = typename SAT::accv_t;
+    __shared__ __attribute__((aligned(16))) unsigned char smem[G::LDS];
+    int* const expo = reinterpret_cast<int*>(smem + G::EXP_OFF);  // [BM] then [BN], per chunk
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int pn = wave % G::PN, pm = wave / G::PN;
+    const int64_t grid = gridDim.x;
+    const int64_t lda = args.lda, ldb = args.ldb, ldc = args.ldc;
+
+    // raw pieces of one chunk: thread t owns 16-byte pieces q = t + i * NT (row q / 16,
+    // k pair q % 16), 16 consecutive lanes per row
+    u4_t xa[G::PA], xb[G::PB];
+    auto load_chunk = [&](int64_t p, int c) {
+        const double* __restrict__ A = args.A + p * args.sa;
+        const double* __restrict__ B = args.B + p * args.sb;
+#pragma unroll
+        for (int i = 0; i < G::PA; ++i) {
+            const int q = tid + i * G::NT;
+            xa[i] = load_raw<16>(A + (q >> 4) * lda + c * G::KC + 2 * (q & 15));
+        }
+#pragma unroll
+        for (int i = 0; i < G::PB; ++i) {
+            const int q = tid + i * G::NT;
+            xb[i] = load_raw<16>(B + (q >> 4) * ldb + c * G::KC + 2 * (q & 15));
+        }
+    };
+    // exponent of the row's chunk maximum, reduced over the row's 16 lanes
+    auto row_exp = [&](const u4_t& raw) {
+        const d2_t x = bitcast<d2_t>(raw);
+        int e = max(exp_of(x[0]), exp_of(x[1]));
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) e = max(e, __shfl_xor(e, off));
+        return e == -4096 ? 0 : e;
+    };
+    auto slice_piece = [&](const u4_t& raw, int row, int kp, int e, unsigned char* plane0) {
+        const d2_t x = bitcast<d2_t>(raw);
+        double r0 = __builtin_amdgcn_ldexp(x[0], G::BETA - e);
+        double r1 = __builtin_amdgcn_ldexp(x[1], G::BETA - e);
+#pragma unroll
+        for (int s = 0; s < G::S; ++s) {
+            const double q0 = __builtin_rint(r0), q1 = __builtin_rint(r1);
+            *reinterpret_cast<uint32_t*>(plane0 + s * G::PLANE + row * G::ROWB + kp * 4) =
+                pack2<SAT>(q0, q1);
+            if (s + 1 < G::S) {
+                r0 = (r0 - q0) * (double)(1 << G::BETA);
+                r1 = (r1 - q1) * (double)(1 << G::BETA);
+            }
+        }
+    };
+
+    int64_t p = blockIdx.x;
+    if (p >= count) return;  // uniform
+    load_chunk(p, 0);
+
+    for (; p < count; p += grid) {
+        double acc[G::IM][G::JN][4];
+#pragma unroll
+        for (int i = 0; i < G::IM; ++i)
+#pragma unroll
+            for (int j = 0; j < G::JN; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[i][j][v] = 0.0;
+
+        for (int c = 0; c < G::NCH; ++c) {
+            // ---- exponents and digit planes of this chunk ----
+#pragma unroll
+            for (int i = 0; i < G::PA; ++i) {
+                const int q = tid + i * G::NT, row = q >> 4, kp = q & 15;
+                const int e = row_exp(xa[i]);
+                if (kp == 0) expo[row] = e;
+                slice_piece(xa[i], row, kp, e, smem);
+            }
+#pragma unroll
+            for (int i = 0; i < G::PB; ++i) {
+                const int q = tid + i * G::NT, row = q >> 4, kp = q & 15;
+                const int e = row_exp(xb[i]);
+                if (kp == 0) expo[G::BM + row] = e;
+                slice_piece(xb[i], row, kp, e, smem + G::SA_BYTES);
+            }
+            // the next chunk is in flight during the products
+            if (c + 1 < G::NCH)
+                load_chunk(p, c + 1);
+            else if (p + grid < count)
+                load_chunk(p + grid, 0);
+            __syncthreads();
+
+            // ---- level products on MFMA, combined in FP64, scaled into acc ----
+            // One 16 x 16 atom at a time: every level's sum of the atom is live, the A
+            // digit fragments of the atom row stay in registers across its columns, and
+            // each B digit plane is read once per atom (plane t feeds levels t + 1 ...).
+#pragma unroll
+            for (int im = 0; im < G::IM; ++im) {
+                frag_t af[G::S];
+#pragma unroll
+                for (int s = 0; s < G::S; ++s)
+                    af[s] = bitcast<frag_t>(load_raw<16>(
+                        smem + s * G::PLANE + (pm * G::TM + im * 16 + (lane & 15)) * G::ROWB +
+                        16 * (lane >> 4)));
+                int erow[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) erow[v] = expo[pm * G::TM + im * 16 + 4 * (lane >> 4) + v];
+#pragma unroll
+                for (int jn = 0; jn < G::JN; ++jn) {
+                    accv_t T[G::G - 1];  // level g in T[g - 2]
+#pragma unroll
+                    for (int l = 0; l < G::G - 1; ++l) T[l] = accv_t{0, 0, 0, 0};
+#pragma unroll
+                    for (int t = 1; t <= G::S; ++t) {
+                        if (t + 1 > G::G) continue;
+                        const frag_t bf = bitcast<frag_t>(load_raw<16>(
+                            smem + G::SA_BYTES + (t - 1) * G::PLANE +
+                            (pn * G::TN + jn * 16 + (lane & 15)) * G::ROWB + 16 * (lane >> 4)));
+#pragma unroll
+                        for (int s = 1; s <= G::S; ++s)
+                            if (s + t <= G::G) SAT::mma(T[s + t - 2], af[s - 1], bf);
+                    }
+                    const int ecol = expo[G::BM + pn * G::TN + jn * 16 + (lane & 15)];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        // smallest weights first
+                        double cs = (double)T[G::G - 2][v] * __builtin_amdgcn_ldexp(1.0, -G::BETA * G::G);
+#pragma unroll
+                        for (int g = G::G - 1; g >= 2; --g)
+                            cs = __builtin_fma((double)T[g - 2][v],
+                                               __builtin_amdgcn_ldexp(1.0, -G::BETA * g), cs);
+                        acc[im][jn][v] += __builtin_amdgcn_ldexp(cs, erow[v] + ecol);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            __syncthreads();  // planes and exponents are rewritten by the next chunk
+        }
+
+        // ---- store ----
+        double* __restrict__ C = args.C + p * args.sc;
+#pragma unroll
+        for (int im = 0; im < G::IM; ++im)
+#pragma unroll
+            for (int jn = 0; jn < G::JN; ++jn) {
+                const int col = pn * G::TN + jn * 16 + (lane & 15);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = pm * G::TM + im * 16 + 4 * (lane >> 4) + v;
+                    double* dst = C + row * ldc + col;
+                    double val = acc[im][jn][v] * args.alpha;
+                    if (args.beta != 0.0) val += args.beta * (*dst);
+                    *dst = val;
+                }
+            }
+    }
+}
+
 template <class SAT, class G>
 hipError_t launch_split(const SplitArgs& a, int64_t count, hipStream_t s) {
     hipLaunchKernelGGL((split_gemm_kernel<SAT, G>), dim3((unsigned)count), dim3(G::NT), 0, s, a);
+    return hipGetLastError();
+}
+
+// Persistent grid: every CU filled to the kernel's occupancy, never more workgroups
+// than problems.
+template <class SAT, class G>
+hipError_t launch_split_pipe(const SplitArgs& a, int64_t count, hipStream_t s) {
+    static int per_cu = -1, cus = 0;
+    if (per_cu < 0) {
+        int dev = 0, n = 0, occ = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, split_pipe_kernel<SAT, G>, G::NT, 0);
+        if (e != hipSuccess) return e;
+        cus = n;
+        per_cu = occ > 0 ? occ : 1;
+    }
+    const int64_t grid = std::min<int64_t>(count, (int64_t)cus * per_cu);
+    hipLaunchKernelGGL((split_pipe_kernel<SAT, G>), dim3((unsigned)grid), dim3(G::NT), 0, s, a,
+                       count);
     return hipGetLastError();
 }
 
@@ -305,13 +491,16 @@ typedef hipError_t (*split_launch_fn)(const SplitArgs&, int64_t, hipStream_t);
 struct SplitInstance {
     int prec, m, n, k, slices, pm, pn, threads, lds;
     split_launch_fn launch;
+    int pipe;  // 1: one workgroup per problem, exponents over full K; 2: persistent
 };
 
 template <class SAT, int PREC, int BM, int BN, int BK, int PM, int PN, int BETA, int S, int G>
 static void add_split(std::vector<SplitInstance>& v) {
     using Geo = SplitGeom<SAT, BM, BN, BK, PM, PN, BETA, S, G>;
     v.push_back(SplitInstance{PREC, BM, BN, BK, S, PM, PN, Geo::NT, Geo::LDS,
-                              &launch_split<SAT, Geo>});
+                              &launch_split<SAT, Geo>, 1});
+    v.push_back(SplitInstance{PREC, BM, BN, BK, S, PM, PN, Geo::NT, Geo::LDS,
+                              &launch_split_pipe<SAT, Geo>, 2});
 }
 
 static const std::vector<SplitInstance>& split_registry() {
@@ -330,11 +519,13 @@ static const std::vector<SplitInstance>& split_registry() {
     return v;
 }
 
+// pipeline 0 selects the persistent form.
 static const SplitInstance* resolve_split(const kami_gemm_desc* d) {
     const int want = d->slices ? d->slices : (d->precision == KAMI_PREC_F64_SPLIT_F16 ? 6 : 7);
+    const int pipe = d->pipeline ? d->pipeline : 2;
     for (const auto& in : split_registry())
         if (in.prec == d->precision && in.m == d->m && in.n == d->n && in.k == d->k &&
-            in.slices == want)
+            in.slices == want && in.pipe == pipe)
             return &in;
     return nullptr;
 }
@@ -354,6 +545,8 @@ kami_status split_query(const kami_gemm_desc* d, kami_gemm_config* out) {
     out->lds_bytes = in->lds;
     out->slices = in->slices;
     out->instance = (int32_t)(in - split_registry().data());
+    out->pipeline = in->pipe;
+    out->prefetch = 1;
     return KAMI_OK;
 }
 

@@ -18,6 +18,8 @@ torch = pytest.importorskip("torch")
 
 SPLITS = [(K.PREC_F64_SPLIT_F16, 6), (K.PREC_F64_SPLIT_F16, 7), (K.PREC_F64_SPLIT_BF16, 7)]
 SIDS = ["f16-s6", "f16-s7", "bf16-s7"]
+# 1: one workgroup per problem, exponents over the full K; 2: persistent, per-chunk exponents
+PIPES = [1, 2]
 
 
 @pytest.fixture(autouse=True)
@@ -25,10 +27,12 @@ def _announce(request):
     print(f"\n[case] {request.node.nodeid}", flush=True)
 
 
-def run(A, B, prec, slices, dev, C=None, **kw):
+def run(A, B, prec, slices, dev, C=None, pipe=0, **kw):
     Ad = torch.from_numpy(np.ascontiguousarray(A)).to(dev)
     Bd = torch.from_numpy(np.ascontiguousarray(B)).to(dev)
     Cd = None if C is None else torch.from_numpy(np.ascontiguousarray(C)).to(dev)
+    if prec in (K.PREC_F64_SPLIT_F16, K.PREC_F64_SPLIT_BF16):
+        kw["pipeline"] = pipe
     out = K.block_gemm(Ad, Bd, Cd, precision=prec, slices=slices, **kw)
     return out.cpu().numpy()
 
@@ -50,20 +54,22 @@ def sizes_for(prec, slices):
 
 
 @pytest.mark.parametrize("prec,slices", SPLITS, ids=SIDS)
-def test_ramp_kat_exact(prec, slices, dev):
+@pytest.mark.parametrize("pipe", PIPES)
+def test_ramp_kat_exact(prec, slices, pipe, dev):
     for s in sizes_for(prec, slices):
         A = np.tile(oracle.ramp(s * s).reshape(1, s, s), (2, 1, 1))
-        C = run(A, A, prec, slices, dev)
+        C = run(A, A, prec, slices, dev, pipe=pipe)
         np.testing.assert_array_equal(C, oracle.gemm_f64(A, A))
 
 
 @pytest.mark.parametrize("prec,slices", SPLITS, ids=SIDS)
-def test_randr_kat_exact(prec, slices, dev):
+@pytest.mark.parametrize("pipe", PIPES)
+def test_randr_kat_exact(prec, slices, pipe, dev):
     for s in sizes_for(prec, slices):
         count = 9
         A = oracle.randr_mod3(count * s * s).reshape(count, s, s) - 1.0
         B = oracle.randr_mod3(2 * count * s * s)[count * s * s:].reshape(count, s, s) * 3.0
-        C = run(A, B, prec, slices, dev)
+        C = run(A, B, prec, slices, dev, pipe=pipe)
         np.testing.assert_array_equal(C, oracle.gemm_f64(A, B))
 
 
@@ -72,14 +78,15 @@ def log_uniform(rng, shape):
 
 
 @pytest.mark.parametrize("prec,slices", SPLITS, ids=SIDS)
-def test_log_uniform_normwise_1e13(prec, slices, dev):
+@pytest.mark.parametrize("pipe", PIPES)
+def test_log_uniform_normwise_1e13(prec, slices, pipe, dev):
     """The split-precision workload: sign-random entries, log10|x| ~ U[-4, 4]."""
     rng = np.random.default_rng(0x4B414D49 + 3)
     for s in sizes_for(prec, slices):
         count = 96
         A = log_uniform(rng, (count, s, s))
         B = log_uniform(rng, (count, s, s))
-        C = run(A, B, prec, slices, dev)
+        C = run(A, B, prec, slices, dev, pipe=pipe)
         exact, absC = oracle.gemm_dot2(A, B)
         err = normwise(C, exact)
         print(f"  {s}^3 normwise {err:.3e}  componentwise "
@@ -90,7 +97,8 @@ def test_log_uniform_normwise_1e13(prec, slices, dev):
 
 
 @pytest.mark.parametrize("prec,slices", SPLITS, ids=SIDS)
-def test_adversarial_rows(prec, slices, dev):
+@pytest.mark.parametrize("pipe", PIPES)
+def test_adversarial_rows(prec, slices, pipe, dev):
     """Zero rows, a single huge entry among tiny ones, signed zeros, exact powers of two,
     entries 2^-40 below their row maximum, all-equal rows and cancelling pairs."""
     rng = np.random.default_rng(17)
@@ -110,21 +118,35 @@ def test_adversarial_rows(prec, slices, dev):
     B[2, 8, :] = -1.0                                 # cancelling columns
     B[3] *= 1e200                                     # large exponents
     A[3] *= 1e-200                                    # small exponents
-    C = run(A, B, prec, slices, dev)
+    C = run(A, B, prec, slices, dev, pipe=pipe)
     exact, _ = oracle.gemm_dot2(A, B)
     assert normwise(C, exact) <= 1e-13
     assert np.all(C[0, 0] == 0.0) and np.all(C[0, 1] == 0.0)
 
 
 @pytest.mark.parametrize("prec,slices", SPLITS[:1] + SPLITS[2:], ids=[SIDS[0], SIDS[2]])
-def test_alpha_beta(prec, slices, dev):
+@pytest.mark.parametrize("pipe", PIPES)
+def test_alpha_beta(prec, slices, pipe, dev):
     s = 64
     rng = np.random.default_rng(3)
     A = rng.integers(-4, 5, (3, s, s)).astype(np.float64)
     B = rng.integers(-4, 5, (3, s, s)).astype(np.float64)
     C0 = rng.integers(-9, 10, (3, s, s)).astype(np.float64)
-    C = run(A, B, prec, slices, dev, C=C0, alpha=-0.5, beta=2.0)
+    C = run(A, B, prec, slices, dev, C=C0, pipe=pipe, alpha=-0.5, beta=2.0)
     np.testing.assert_array_equal(C, -0.5 * oracle.gemm_f64(A, B) + 2.0 * C0)
+
+
+@pytest.mark.parametrize("prec,slices", SPLITS, ids=SIDS)
+def test_persistent_walks_many_problems(prec, slices, dev):
+    """More problems than resident workgroups, so each persistent workgroup handles
+    several problems and prefetches across problem boundaries; integer inputs make
+    every digit product exact, so the result must equal the FP64 product."""
+    for s in sizes_for(prec, slices):
+        count = 1100 if s == 64 else 601
+        A = oracle.randr_mod3(count * s * s).reshape(count, s, s) - 1.0
+        B = (oracle.randr_mod3(2 * count * s * s)[count * s * s:].reshape(count, s, s) - 1.0) * 5.0
+        C = run(A, B, prec, slices, dev, pipe=2)
+        np.testing.assert_array_equal(C, oracle.gemm_f64(A, B))
 
 
 def test_split_unsupported_requests_are_enotsup():
