@@ -58,6 +58,14 @@ constexpr bool levels_exact(int beta, int S, int G, int kc) {
     return true;
 }
 
+// 2^e as a compile-time constant (exact for the exponents used here).
+constexpr double pow2(int e) {
+    double r = 1.0;
+    for (; e > 0; --e) r *= 2.0;
+    for (; e < 0; ++e) r *= 0.5;
+    return r;
+}
+
 template <class SAT, int BM_, int BN_, int BK_, int PM_, int PN_, int BETA_, int S_, int G_>
 struct SplitGeom {
     static constexpr int BM = BM_, BN = BN_, BK = BK_, PM = PM_, PN = PN_;
@@ -93,14 +101,27 @@ __device__ inline int exp_of(double x) {
     return x == 0.0 ? -4096 : __builtin_amdgcn_frexp_exp(x);
 }
 
+// Maximum over the 16 lanes of a DPP row (lanes 16r .. 16r + 15), by lane permutations
+// within the row: swap neighbours, swap pairs, mirror each half row, mirror the row.
+// An all-zero row (sentinel exponent) maps to 0.
+__device__ inline int row16_max_exp(int e) {
+    e = max(e, __builtin_amdgcn_update_dpp(0, e, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    e = max(e, __builtin_amdgcn_update_dpp(0, e, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    e = max(e, __builtin_amdgcn_update_dpp(0, e, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    e = max(e, __builtin_amdgcn_update_dpp(0, e, 0x140, 0xF, 0xF, false));  // row_mirror
+    return e == -4096 ? 0 : e;
+}
+
 template <class SAT>
 __device__ inline uint32_t pack2(double a, double b);
 
 template <>
 __device__ inline uint32_t pack2<AtomF16x32>(double a, double b) {
-    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-    h2 v = {(_Float16)(float)a, (_Float16)(float)b};  // small integers: exact
-    return __builtin_bit_cast(uint32_t, v);
+    // Small integers: exact in float and in half.  The conversion goes through float
+    // explicitly (f64 -> f32, then one packed f32 -> f16 pair conversion, exact here): a
+    // direct double -> half conversion has no instruction and is emulated in integer code.
+    const float fa = (float)a, fb = (float)b;
+    return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(fa, fb));
 }
 
 template <>
@@ -129,7 +150,7 @@ split_gemm_kernel(SplitArgs args) {
 
     // ---- 1. row exponents of A and B.  Thread t owns the 16-byte pieces
     //      p = t + i * NT of every K chunk (row p / 16, k pair p % 16), so the 16 lanes
-    //      of a row reduce with four shuffles; every chunk's loads are independent. ----
+    //      of a row reduce within one DPP row; every chunk's loads are independent. ----
     int ea[G::PA], eb[G::PB];
 #pragma unroll
     for (int i = 0; i < G::PA; ++i) ea[i] = -4096;
@@ -151,21 +172,16 @@ split_gemm_kernel(SplitArgs args) {
         }
     }
 #pragma unroll
-    for (int off = 1; off < 16; off <<= 1) {
+    for (int i = 0; i < G::PA; ++i) ea[i] = row16_max_exp(ea[i]);
 #pragma unroll
-        for (int i = 0; i < G::PA; ++i) ea[i] = max(ea[i], __shfl_xor(ea[i], off));
-#pragma unroll
-        for (int i = 0; i < G::PB; ++i) eb[i] = max(eb[i], __shfl_xor(eb[i], off));
-    }
+    for (int i = 0; i < G::PB; ++i) eb[i] = row16_max_exp(eb[i]);
 #pragma unroll
     for (int i = 0; i < G::PA; ++i) {
-        if (ea[i] == -4096) ea[i] = 0;
         const int p = tid + i * G::NT;
         if ((p & 15) == 0) expo[p >> 4] = ea[i];
     }
 #pragma unroll
     for (int i = 0; i < G::PB; ++i) {
-        if (eb[i] == -4096) eb[i] = 0;
         const int p = tid + i * G::NT;
         if ((p & 15) == 0) expo[G::BM + (p >> 4)] = eb[i];
     }
@@ -260,7 +276,7 @@ split_gemm_kernel(SplitArgs args) {
                         for (int im = 0; im < G::IM; ++im) SAT::mma(T[im][jh], af[s - 1][im], bf);
                     }
                 }
-                const double w = __builtin_amdgcn_ldexp(1.0, -G::BETA * g);
+                const double w = pow2(-G::BETA * g);
 #pragma unroll
                 for (int im = 0; im < G::IM; ++im)
 #pragma unroll
@@ -336,10 +352,7 @@ split_pipe_kernel(SplitArgs args, int64_t count) {
     // exponent of the row's chunk maximum, reduced over the row's 16 lanes
     auto row_exp = [&](const u4_t& raw) {
         const d2_t x = bitcast<d2_t>(raw);
-        int e = max(exp_of(x[0]), exp_of(x[1]));
-#pragma unroll
-        for (int off = 1; off < 16; off <<= 1) e = max(e, __shfl_xor(e, off));
-        return e == -4096 ? 0 : e;
+        return row16_max_exp(max(exp_of(x[0]), exp_of(x[1])));
     };
     auto slice_piece = [&](const u4_t& raw, int row, int kp, int e, unsigned char* plane0) {
         const d2_t x = bitcast<d2_t>(raw);
@@ -427,11 +440,10 @@ split_pipe_kernel(SplitArgs args, int64_t count) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         // smallest weights first
-                        double cs = (double)T[G::G - 2][v] * __builtin_amdgcn_ldexp(1.0, -G::BETA * G::G);
+                        double cs = (double)T[G::G - 2][v] * pow2(-G::BETA * G::G);
 #pragma unroll
                         for (int g = G::G - 1; g >= 2; --g)
-                            cs = __builtin_fma((double)T[g - 2][v],
-                                               __builtin_amdgcn_ldexp(1.0, -G::BETA * g), cs);
+                            cs = __builtin_fma((double)T[g - 2][v], pow2(-G::BETA * g), cs);
                         acc[im][jn][v] += __builtin_amdgcn_ldexp(cs, erow[v] + ecol);
                     }
                     __builtin_amdgcn_sched_barrier(0);
