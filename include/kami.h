@@ -197,6 +197,18 @@ kami_status kami_bsr_spgemm_numeric(const kami_bsr* A, const kami_bsr* B, const 
                                     const int32_t* colidxC, int64_t nnzbC, void* valC,
                                     int32_t precision, double alpha, void* stream);
 
+/* ---------------------------------------------------------------- block SpMM */
+/* C (mb*bs x n, row-major, ldc) = alpha * A * op(B) + beta * C, A in BSR (K = nb*bs).
+ * B dense: KAMI_B_NK -> B stored n x K row-major (ldb >= K), C(i,j) = sum_k A(i,k) B[j*ldb+k],
+ * the reference's layout; KAMI_B_KN -> B stored K x n (ldb >= n).  ldb / ldc of 0 mean
+ * packed.  n must be a multiple of 16; bs 16 or 32.  precision: KAMI_PREC_F16_F32 /
+ * KAMI_PREC_BF16_F32 (FP16/BF16 A and B, float C) or KAMI_PREC_F64.
+ * Replaces block_spmm_{1d,2d,3d}_half_mma (src/sparse/spmm/2d/block_spmm_2d_half_mma.cu:72)
+ * and its host-side re-tiling bsr2tilebsr (:264-291). */
+kami_status kami_bsr_spmm(const kami_bsr* A, const void* B, int32_t n, int64_t ldb,
+                          int32_t b_layout, void* C, int64_t ldc, int32_t precision,
+                          double alpha, double beta, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -261,4 +261,26 @@ def bsr_spgemm(A: BsrMatrix, B: BsrMatrix, *, precision=None, alpha=1.0, stream=
     return BsrMatrix(A.mb, B.nb, A.bs, rowptr, colidx, vals)
 
 
-__all__ += ["BsrMatrix", "spgemm_symbolic", "spgemm_fill", "spgemm_numeric", "bsr_spgemm"]
+def bsr_spmm(A: BsrMatrix, B, C=None, *, b_layout="nk", precision=None, alpha=1.0, beta=0.0,
+             stream=None):
+    """C = alpha * A * op(B) + beta * C with A block-sparse and B dense.
+
+    b_layout "nk": B is (n, K) row-major and C(i, j) = sum_k A(i, k) B[j, k] (the
+    reference's layout); "kn": B is (K, n).  K = A.nb * A.bs.  C is (A.mb * A.bs, n),
+    float32 for FP16/BF16 inputs, float64 for FP64."""
+    import torch
+    prec = _precision_of(A.vals if A.nnzb else B, precision)
+    kn = b_layout == "kn"
+    n = int(B.shape[1] if kn else B.shape[0])
+    out_dt = torch.float64 if prec == PREC_F64 else torch.float32
+    if C is None:
+        C = torch.zeros((A.mb * A.bs, n), dtype=out_dt, device=B.device)
+    a = A._c()
+    check(lib().kami_bsr_spmm(ctypes.byref(a), ctypes.c_void_p(B.data_ptr()), n, B.stride(0),
+                              B_KN if kn else B_NK, ctypes.c_void_p(C.data_ptr()), C.stride(0),
+                              prec, alpha, beta, _stream_ptr(stream)), "kami_bsr_spmm")
+    return C
+
+
+__all__ += ["BsrMatrix", "spgemm_symbolic", "spgemm_fill", "spgemm_numeric", "bsr_spgemm",
+            "bsr_spmm"]

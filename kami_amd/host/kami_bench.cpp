@@ -10,6 +10,8 @@
 //             per element, one problem per workgroup, uniform sizes)
 //   spgemm  : src/sparse/spgemm/2d/block_spgemm_2d_half_mma.cu:720-870 (srand(0) BSR
 //             generator for A and B, symbolic + numeric, FLOP = 2 nnzCub bs^3)
+//   spmm    : src/sparse/spmm/2d/block_spmm_2d_half_mma.cu:293-427 (srand(0) BSR generator
+//             with values rand() % 2 + 1, dense B n x K, FLOP = 2 nnz(A) n)
 // Shapes, precision and partition are run-time arguments instead of -D macros.  Unlike
 // the reference, a failed validation makes the exit status 1.
 #include <hip/hip_runtime_api.h>
@@ -473,6 +475,93 @@ int run_spgemm(const Args& a) {
     return error;
 }
 
+// The SpMM program (src/sparse/spmm/2d/block_spmm_2d_half_mma.cu:293-427): the BSR
+// generator with values rand() % 2 + 1, then B (n x K) rand() % 2 + 1 continuing the same
+// sequence; FLOP = 2 nnz(A) n with nnz(A) counted in elements.
+int run_spmm(const Args& a) {
+    const int prec = parse_prec(a.get("--prec", "f16"));
+    if (prec == KAMI_PREC_F64_SPLIT_F16 || prec == KAMI_PREC_F64_SPLIT_BF16) {
+        std::fprintf(stderr, "spmm: --prec f16, bf16 or f64\n");
+        return 2;
+    }
+    const int nbr = (int)a.geti("--blocks", 8), bs = (int)a.geti("--bs", 16);
+    const int n = (int)a.geti("--n", nbr * bs);
+    const float sparsity = (float)a.getd("--sparsity", 0.5);
+    print_device();
+    std::srand(0);
+    std::vector<int32_t> rp(nbr + 1, 0), ci;
+    std::vector<double> val;
+    for (int i = 0; i < nbr; ++i) {
+        for (int j = 0; j < nbr; ++j) {
+            const float r = (float)std::rand() / (float)RAND_MAX;
+            if (r > sparsity) {
+                ci.push_back(j);
+                for (int q = 0; q < bs * bs; ++q) val.push_back((double)(std::rand() % 2) + 1.0);
+            }
+        }
+        rp[i + 1] = (int32_t)ci.size();
+    }
+    const int64_t K = (int64_t)nbr * bs;
+    std::vector<double> B((size_t)n * K);
+    for (auto& x : B) x = (double)(std::rand() % 2) + 1.0;
+    const int nnzb = (int)ci.size();
+    int32_t *drp = nullptr, *dci = nullptr;
+    HIP_OK(hipMalloc(&drp, rp.size() * 4));
+    HIP_OK(hipMalloc(&dci, std::max<size_t>(ci.size(), 1) * 4));
+    HIP_OK(hipMemcpy(drp, rp.data(), rp.size() * 4, hipMemcpyHostToDevice));
+    if (!ci.empty()) HIP_OK(hipMemcpy(dci, ci.data(), ci.size() * 4, hipMemcpyHostToDevice));
+    void* dval = upload(val.empty() ? std::vector<double>(1, 0.0) : val, prec);
+    void* dB = upload(B, prec);
+    const size_t eo = half_input(prec) ? 4 : 8;
+    const size_t csz = (size_t)nbr * bs * n;
+    void* dC = nullptr;
+    HIP_OK(hipMalloc(&dC, csz * eo));
+    HIP_OK(hipMemset(dC, 0, csz * eo));
+    kami_bsr A{nbr, nbr, bs, nnzb, drp, dci, dval};
+    std::printf("Launching kernel with %d x %d blocks and 256 threads per block\n", nbr,
+                (n + 63) / 64);
+    KAMI_OK_OR_DIE(kami_bsr_spmm(&A, dB, n, 0, KAMI_B_NK, dC, 0, prec, 1.0, 0.0, nullptr));
+    hipEvent_t t0, t1;
+    HIP_OK(hipEventCreate(&t0));
+    HIP_OK(hipEventCreate(&t1));
+    HIP_OK(hipEventRecord(t0, nullptr));
+    KAMI_OK_OR_DIE(kami_bsr_spmm(&A, dB, n, 0, KAMI_B_NK, dC, 0, prec, 1.0, 0.0, nullptr));
+    HIP_OK(hipEventRecord(t1, nullptr));
+    HIP_OK(hipEventSynchronize(t1));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, t0, t1));
+    const double ops = 2.0 * (double)nnzb * bs * bs * n;
+    const double gflops = ops / (ms * 1e6), tflops = gflops / 1e3;
+    std::printf("Execution time: %g ms\n", ms);
+    std::printf("Performance: %g GFLOPS (%g TFLOPS)\n", gflops, tflops);
+    std::printf("[hemeng_log],2d,%d,%d,%g,%g\n", nbr * bs, 4, ms, tflops);
+
+    std::vector<double> C = download_c(dC, csz, prec);
+    std::vector<double> R(csz, 0.0);
+    for (int i = 0; i < nbr; ++i)
+        for (int q = rp[i]; q < rp[i + 1]; ++q)
+            for (int r = 0; r < bs; ++r)
+                for (int c = 0; c < bs; ++c) {
+                    const double x = val[(size_t)q * bs * bs + r * bs + c];
+                    for (int j = 0; j < n; ++j)
+                        R[((size_t)i * bs + r) * n + j] += x * B[(size_t)j * K + (size_t)ci[q] * bs + c];
+                }
+    int error = 0;
+    for (size_t e = 0; e < csz; ++e)
+        if (std::fabs(C[e] - R[e]) > 1e-6) {
+            std::printf("Error: %zu %.17g %.17g\n", e, C[e], R[e]);
+            error = 1;
+            break;
+        }
+    if (!error) std::printf("Validation successful!\n");
+    HIP_OK(hipFree(drp));
+    HIP_OK(hipFree(dci));
+    HIP_OK(hipFree(dval));
+    HIP_OK(hipFree(dB));
+    HIP_OK(hipFree(dC));
+    return error;
+}
+
 void usage() {
     std::printf(
         "usage: kami_bench block   [--prec f64|f16|bf16|split16|splitbf16] [--m M --n N --k K]\n"
@@ -481,6 +570,8 @@ void usage() {
         "       kami_bench batched [--prec f64|f16] [--m M --n N --k K] [--count C]\n"
         "                          [--iters I] [--graph 0|1] [--autotune 0|1]\n"
         "       kami_bench spgemm  [--prec f16|bf16|f64] [--blocks B] [--bs 16|32]\n"
+        "                          [--sparsity S]\n"
+        "       kami_bench spmm    [--prec f16|bf16|f64] [--blocks B] [--bs 16|32] [--n N]\n"
         "                          [--sparsity S]\n");
 }
 
@@ -497,6 +588,7 @@ int main(int argc, char** argv) {
     if (cmd == "block") return run_block(a);
     if (cmd == "batched") return run_batched(a);
     if (cmd == "spgemm") return run_spgemm(a);
+    if (cmd == "spmm") return run_spmm(a);
     usage();
     return 2;
 }

@@ -69,6 +69,11 @@ def lib():
             "oracle_spgemm_numeric_f64": (None, [_c.c_int, _c.c_int, _c.c_int, _i32p, _i32p, _f64p,
                                                  _i32p, _i32p, _f64p, _i32p, _f64p]),
             "oracle_spgemm_nnzcub": (_c.c_int64, [_c.c_int, _i32p, _i32p]),
+            "oracle_bsr_spmm_f64": (None, [_c.c_int, _c.c_int, _c.c_int, _i32p, _i32p, _f64p,
+                                           _f64p, _c.c_int, _c.c_int64, _c.c_int, _f64p,
+                                           _c.c_int64]),
+            "oracle_spmm_generate": (_c.c_int, [_c.c_int, _c.c_int, _c.c_float, _c.c_int, _i32p,
+                                                _i32p, _f64p, _c.c_int, _f64p]),
             "oracle_max_threads": (_c.c_int, []),
         }
         for name, (res, args) in sig.items():
@@ -214,3 +219,25 @@ def spgemm_nnzcub(ciA, rpB):
 
 def max_threads():
     return int(lib().oracle_max_threads())
+
+
+def bsr_spmm(mb, nb, bs, rp, ci, vals, B, b_kn=False):
+    """C (mb*bs, n) = A_bsr * op(B) in double; vals (nnzb, bs, bs) float64, B (n, K) or
+    (K, n) for b_kn."""
+    B = _c64(B)
+    n = B.shape[1] if b_kn else B.shape[0]
+    C = np.zeros((mb * bs, n), dtype=np.float64)
+    lib().oracle_bsr_spmm_f64(mb, nb, bs, np.ascontiguousarray(rp, np.int32),
+                              np.ascontiguousarray(ci, np.int32), _c64(vals).ravel(), B.ravel(),
+                              n, B.shape[1], int(bool(b_kn)), C.ravel(), n)
+    return C
+
+
+def spmm_generate(rows, cols, sparsity, bs, n):
+    """The reference SpMM program's inputs: (rowptr, colidx, vals (nnzb, bs, bs), B (n, K))."""
+    rp = np.zeros(rows + 1, dtype=np.int32)
+    ci = np.zeros(rows * cols, dtype=np.int32)
+    vals = np.zeros(rows * cols * bs * bs, dtype=np.float64)
+    B = np.zeros((n, cols * bs), dtype=np.float64)
+    nnzb = lib().oracle_spmm_generate(rows, cols, sparsity, bs, rp, ci, vals, n, B.ravel())
+    return rp, ci[:nnzb].copy(), vals[: nnzb * bs * bs].reshape(nnzb, bs, bs).copy(), B

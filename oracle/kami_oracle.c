@@ -400,3 +400,59 @@ int oracle_max_threads(void) {
     return 1;
 #endif
 }
+
+/* Block SpMM check loop (src/sparse/spmm/2d/block_spmm_2d_half_mma.cu:378-403) on plain BSR
+ * (the reference walks its per-rank re-tiling of the same blocks): for every stored block
+ * (i, kb) and every r, c of the block and column j of C,
+ *   C[(i bs + r) ldc + j] += A(i, kb)[r][c] * B[j ldb + kb bs + c]       (b_kn = 0, B n x K)
+ *   C[(i bs + r) ldc + j] += A(i, kb)[r][c] * B[(kb bs + c) ldb + j]     (b_kn = 1, B K x n)
+ * in double.  C is accumulated into (the caller zeroes it). */
+void oracle_bsr_spmm_f64(int mb, int nb, int bs, const int32_t* rp, const int32_t* ci,
+                         const double* vals, const double* B, int n, int64_t ldb, int b_kn,
+                         double* C, int64_t ldc) {
+    (void)nb;
+    const int64_t b2 = (int64_t)bs * bs;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+    for (int i = 0; i < mb; i++)
+        for (int q = rp[i]; q < rp[i + 1]; q++) {
+            const double* a = vals + (int64_t)q * b2;
+            const int64_t kb = ci[q];
+            for (int r = 0; r < bs; r++)
+                for (int c = 0; c < bs; c++) {
+                    const double x = a[r * bs + c];
+                    double* crow = C + ((int64_t)i * bs + r) * ldc;
+                    for (int j = 0; j < n; j++) {
+                        const double y = b_kn ? B[(kb * bs + c) * ldb + j] : B[j * ldb + kb * bs + c];
+                        crow[j] += x * y;
+                    }
+                }
+        }
+}
+
+/* The SpMM program's inputs (src/sparse/spmm/2d/block_spmm_2d_half_mma.cu:237-262 and
+ * :315-320): srand(0); a block is kept when rand()/RAND_MAX > sparsity and its values are
+ * rand() % 2 + 1; then the n x K dense B takes rand() % 2 + 1 continuing the same
+ * sequence.  Returns the number of stored blocks. */
+int oracle_spmm_generate(int rows, int cols, float sparsity, int bs, int32_t* rp, int32_t* ci,
+                         double* vals, int n, double* B) {
+    int count = 0;
+    srand(0);
+    rp[0] = 0;
+    for (int i = 0; i < rows; i++) {
+        for (int j = 0; j < cols; j++) {
+            float r = (float)rand() / (float)RAND_MAX;
+            if (r > sparsity) {
+                ci[count] = j;
+                for (int len = 0; len < bs * bs; len++)
+                    vals[(int64_t)count * bs * bs + len] = (double)(rand() % 2) + 1.0;
+                count++;
+            }
+        }
+        rp[i + 1] = count;
+    }
+    const int64_t nb_elems = (int64_t)n * cols * bs;
+    for (int64_t i = 0; i < nb_elems; i++) B[i] = (double)(rand() % 2) + 1.0;
+    return count;
+}

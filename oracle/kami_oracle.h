@@ -45,5 +45,10 @@ void oracle_spgemm_numeric_f64(int mb, int nb, int bs, const int32_t* rpA, const
                                const double* vA, const int32_t* rpB, const int32_t* ciB,
                                const double* vB, const int32_t* rpC, double* vC);
 int64_t oracle_spgemm_nnzcub(int nnzbA, const int32_t* ciA, const int32_t* rpB);
+void oracle_bsr_spmm_f64(int mb, int nb, int bs, const int32_t* rp, const int32_t* ci,
+                         const double* vals, const double* B, int n, int64_t ldb, int b_kn,
+                         double* C, int64_t ldc);
+int oracle_spmm_generate(int rows, int cols, float sparsity, int bs, int32_t* rp, int32_t* ci,
+                         double* vals, int n, double* B);
 int oracle_max_threads(void);
 #endif

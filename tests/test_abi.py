@@ -135,3 +135,33 @@ def test_batched_plan_validation_without_device():
                                         ctypes.byref(h)) == _abi.ENOTSUP
     assert lib.kami_batched_plan_create(arr, 0, K.PREC_F64, ctypes.byref(opts), None,
                                         ctypes.byref(h)) == _abi.EINVAL
+
+
+def test_spmm_validation_without_device():
+    lib = K.lib()
+    rp = (ctypes.c_int32 * 3)(0, 1, 1)
+    ci = (ctypes.c_int32 * 1)(0)
+    A = _abi.Bsr()
+    A.mb, A.nb, A.bs, A.nnzb = 2, 2, 16, 1
+    A.rowptr = ctypes.addressof(rp)
+    A.colidx = ctypes.addressof(ci)
+    A.vals = 1 << 20
+    B, C = 1 << 21, 1 << 22
+
+    def call(n=32, ldb=0, layout=K.B_NK, ldc=0, prec=K.PREC_F16_F32, b=B, c=C):
+        return lib.kami_bsr_spmm(ctypes.byref(A), b, n, ldb, layout, c, ldc, prec, 1.0, 0.0, None)
+
+    assert call(n=24) == _abi.EINVAL          # n not a multiple of 16
+    assert call(n=0) == _abi.EINVAL
+    assert call(b=None) == _abi.EINVAL
+    assert call(c=None) == _abi.EINVAL
+    assert call(layout=7) == _abi.EINVAL
+    assert call(ldb=16) == _abi.EINVAL        # n x K layout needs ldb >= K = 32
+    assert call(layout=K.B_KN, ldb=16) == _abi.EINVAL
+    assert call(ldc=16) == _abi.EINVAL
+    assert call(ldb=33) == _abi.EINVAL        # 66-byte rows break 16-byte alignment
+    assert call(prec=K.PREC_F64_SPLIT_F16) == _abi.ENOTSUP
+    A.bs = 8
+    assert call() == _abi.ENOTSUP
+    A.bs = 16
+    assert call() == _abi.ENODEV              # valid request, no GPU here

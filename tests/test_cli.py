@@ -22,7 +22,7 @@ def test_cli_is_built_and_prints_usage():
     assert os.access(CLI, os.X_OK), "build kami_amd/csrc first"
     r = run("--help")
     assert r.returncode == 0
-    for cmd in ("block", "batched", "spgemm"):
+    for cmd in ("block", "batched", "spgemm", "spmm"):
         assert f"kami_bench {cmd}" in r.stdout
 
 
@@ -48,6 +48,9 @@ CASES = [
     (["spgemm", "--prec", "f16", "--blocks", "8", "--bs", "16"], r"\[hemeng_log\],2d,\d+,128,128,128,"),
     (["spgemm", "--prec", "f64", "--blocks", "24", "--bs", "32", "--sparsity", "0.8"],
      r"\[hemeng_log\],2d,\d+,768,768,768,"),
+    (["spmm", "--prec", "f16", "--blocks", "8", "--bs", "16"], r"\[hemeng_log\],2d,128,4,"),
+    (["spmm", "--prec", "f64", "--blocks", "12", "--bs", "32", "--n", "80"],
+     r"\[hemeng_log\],2d,384,4,"),
 ]
 
 

@@ -168,6 +168,48 @@ def test_glibc_rand_known_answer_drives_bsr_generator():
     assert set(np.unique(v)) <= {1.0, 2.0, 3.0}
 
 
+def test_spmm_generator_follows_glibc_sequence():
+    """The SpMM program's generator: block (0, 0) is kept by the first draw, its values are
+    rand() % 2 + 1 of the next draws (846930886, 1681692777, 1714636915, 1957747793), and B
+    continues the same sequence."""
+    rp, ci, vals, B = oracle.spmm_generate(4, 4, 0.5, 16, 32)
+    assert ci[0] == 0
+    assert list(vals[0, 0, :4]) == [846930886 % 2 + 1, 1681692777 % 2 + 1,
+                                    1714636915 % 2 + 1, 1957747793 % 2 + 1]
+    assert set(np.unique(vals)) <= {1.0, 2.0} and set(np.unique(B)) <= {1.0, 2.0}
+    assert B.shape == (32, 64)
+    rp2, ci2, _ = oracle.bsr_generate(4, 4, 0.5, 16)
+    np.testing.assert_array_equal(rp, rp2)  # same keep decisions as the SpGEMM generator
+    np.testing.assert_array_equal(ci, ci2)
+
+
+@pytest.mark.parametrize("b_kn", [False, True])
+def test_bsr_spmm_oracle_vs_dense_product(b_kn):
+    """The restated check loop equals the dense product of the expanded matrix, in both B
+    layouts, on integer values (exact in any order) and within rounding on random ones."""
+    rng = np.random.default_rng(21)
+    for mb, nb, bs, n, dens in [(3, 5, 16, 48, 0.4), (6, 4, 32, 16, 0.7), (1, 1, 16, 16, 1.0),
+                                (4, 3, 16, 32, 0.0)]:
+        mask = rng.random((mb, nb)) < dens
+        rp = np.concatenate([[0], np.cumsum(mask.sum(1))]).astype(np.int32)
+        ci = np.nonzero(mask)[1].astype(np.int32)
+        for integer in (True, False):
+            vals = (rng.integers(-3, 4, (len(ci), bs, bs)) if integer
+                    else rng.standard_normal((len(ci), bs, bs))).astype(np.float64)
+            Bnk = (rng.integers(-3, 4, (n, nb * bs)) if integer
+                   else rng.standard_normal((n, nb * bs))).astype(np.float64)
+            dense = np.zeros((mb * bs, nb * bs))
+            for i in range(mb):
+                for q in range(rp[i], rp[i + 1]):
+                    dense[i * bs:(i + 1) * bs, ci[q] * bs:(ci[q] + 1) * bs] = vals[q]
+            Bin = np.ascontiguousarray(Bnk.T) if b_kn else Bnk
+            C = oracle.bsr_spmm(mb, nb, bs, rp, ci, vals, Bin, b_kn=b_kn)
+            if integer:
+                np.testing.assert_array_equal(C, dense @ Bnk.T)
+            else:
+                np.testing.assert_allclose(C, dense @ Bnk.T, rtol=1e-12, atol=1e-12)
+
+
 def test_exclusive_scan_vs_cumsum():
     rng = np.random.default_rng(9)
     for n in [0, 1, 2, 7, 257]:
