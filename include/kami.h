@@ -197,6 +197,17 @@ kami_status kami_bsr_spgemm_numeric(const kami_bsr* A, const kami_bsr* B, const 
                                     const int32_t* colidxC, int64_t nnzbC, void* valC,
                                     int32_t precision, double alpha, void* stream);
 
+/* ---------------------------------------------------------------- tuning cache */
+/* kami_block_gemm_autotune and batched plans created with autotune record their choices
+ * per (precision, M, N, K) and per (precision, size bucket).  Save writes them to a text
+ * file; load merges a file (all or nothing: KAMI_EINVAL on any malformed record or on a
+ * block choice that names no compiled instance); clear forgets them.  Replaces the
+ * per-size tables of the reference's sweep scripts (scripts/fp64_block_square_GH200.sh:11-25,
+ * scripts/fp64_batched_GH200.sh:96-123). */
+kami_status kami_tune_save(const char* path);
+kami_status kami_tune_load(const char* path);
+kami_status kami_tune_clear(void);
+
 /* ---------------------------------------------------------------- block SpMM */
 /* C (mb*bs x n, row-major, ldc) = alpha * A * op(B) + beta * C, A in BSR (K = nb*bs).
  * B dense: KAMI_B_NK -> B stored n x K row-major (ldb >= K), C(i,j) = sum_k A(i,k) B[j*ldb+k],

@@ -7,6 +7,7 @@ library's HIP kernels.
 from __future__ import annotations
 
 import ctypes
+import os
 
 from . import _abi
 from ._abi import (B_KN, B_NK, PART_1D, PART_2D, PART_3D, PART_AUTO, PREC_BF16_F32,
@@ -284,3 +285,20 @@ def bsr_spmm(A: BsrMatrix, B, C=None, *, b_layout="nk", precision=None, alpha=1.
 
 __all__ += ["BsrMatrix", "spgemm_symbolic", "spgemm_fill", "spgemm_numeric", "bsr_spgemm",
             "bsr_spmm"]
+
+
+def tune_save(path):
+    """Write the recorded block-GEMM and batched tuning choices to `path`."""
+    check(lib().kami_tune_save(os.fsencode(path)), "kami_tune_save")
+
+
+def tune_load(path):
+    """Merge the tuning choices in `path` (all or nothing)."""
+    check(lib().kami_tune_load(os.fsencode(path)), "kami_tune_load")
+
+
+def tune_clear():
+    check(lib().kami_tune_clear(), "kami_tune_clear")
+
+
+__all__ += ["tune_save", "tune_load", "tune_clear"]

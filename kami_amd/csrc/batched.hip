@@ -517,7 +517,10 @@ kami_status kami_batched_plan_create(const kami_problem* probs, int64_t count, i
         bucket[i] = bucket_of(hp[i]);
         used[bucket[i]] = true;
     }
-    for (int b = 0; b < 125; ++b) variant_of_bucket[b] = prior_variant(b);
+    for (int b = 0; b < 125; ++b) {
+        const int tuned = batched_tuned_variant(precision, b);
+        variant_of_bucket[b] = tuned >= 0 ? tuned : prior_variant(b);
+    }
 
     hipError_t e = persistent_grid(precision, &plan->grid);
     if (e != hipSuccess) {
@@ -581,6 +584,7 @@ kami_status kami_batched_plan_create(const kami_problem* probs, int64_t count, i
                 }
             }
             (void)hipFree(d_sample);
+            if (st == KAMI_OK) set_batched_tuned_variant(precision, b, variant_of_bucket[b]);
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);

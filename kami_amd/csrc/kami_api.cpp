@@ -123,6 +123,20 @@ void set_tuned_choice(int prec, int m, int n, int k, int part, int ranks, int pi
     g_tuned.push_back({TuneKey{prec, m, n, k}, TuneVal{part, ranks, pipe}});
 }
 
+std::vector<TunedBlock> tuned_block_choices() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::vector<TunedBlock> v;
+    for (auto& kv : g_tuned)
+        v.push_back(TunedBlock{kv.first.prec, kv.first.m, kv.first.n, kv.first.k, kv.second.part,
+                               kv.second.ranks, kv.second.pipe});
+    return v;
+}
+
+void clear_tuned_block_choices() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned.clear();
+}
+
 static bool get_tuned_choice(int prec, int m, int n, int k, TuneVal* out) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (auto& kv : g_tuned)

@@ -18,6 +18,18 @@ const std::vector<BlockInstance>& registry();
 kami_status validate_desc(const kami_gemm_desc* d);
 const BlockInstance* resolve_block(const kami_gemm_desc* d, kami_status* st);
 void set_tuned_choice(int prec, int m, int n, int k, int part, int ranks, int pipe);
+// Block-GEMM choices recorded by the autotuner (or loaded from a tuning file).
+struct TunedBlock {
+    int prec, m, n, k, part, ranks, pipe;
+};
+std::vector<TunedBlock> tuned_block_choices();
+void clear_tuned_block_choices();
+// Batched tile variant recorded per (precision, size bucket); -1 when none.
+constexpr int kBatchedBuckets = 125;
+constexpr int kBatchedVariants = 4;
+int batched_tuned_variant(int prec, int bucket);
+void set_batched_tuned_variant(int prec, int bucket, int variant);
+void clear_batched_tuned_variants();
 int elt_in(int prec);
 int elt_out(int prec);
 
