@@ -100,3 +100,32 @@ def test_normal_values_within_tolerance(prec, dev):
     ref = oracle.bsr_spmm(mb, nb, bs, rp, ci, vals_c, B_c)
     absref = oracle.bsr_spmm(mb, nb, bs, rp, ci, np.abs(vals_c), np.abs(B_c))
     assert np.all(np.abs(C - ref) <= 1e-2 * absref + 1e-30)
+
+
+def test_matrix_market_input_through_spgemm_and_spmm(tmp_path, dev):
+    """A Matrix Market file read into BSR drives both sparse entry points; integer values
+    make the FP64 results exact."""
+    import scipy.io
+    import scipy.sparse as sp
+
+    from kami_amd import mtx
+
+    rng = np.random.default_rng(11)
+    M = sp.random(300, 300, density=0.01, random_state=rng, format="coo",
+                  data_rvs=lambda k: rng.integers(-3, 4, k).astype(np.float64))
+    f = tmp_path / "m.mtx"
+    scipy.io.mmwrite(str(f), M)
+    mb, nb, rp, ci, blocks = mtx.load_matrix_market_bsr(f, 16)
+    dense = np.zeros((mb * 16, nb * 16))
+    dense[:300, :300] = M.toarray()
+    A = to_dev(rp, ci, blocks, mb, nb, 16, torch.float64, dev)
+    C = K.bsr_spgemm(A, A, precision=K.PREC_F64)
+    got = np.zeros_like(dense)
+    rpc, cic, vc = C.rowptr.cpu().numpy(), C.colidx.cpu().numpy(), C.vals.cpu().numpy()
+    for i in range(mb):
+        for q in range(rpc[i], rpc[i + 1]):
+            got[i * 16:(i + 1) * 16, cic[q] * 16:(cic[q] + 1) * 16] = vc[q]
+    np.testing.assert_array_equal(got, dense @ dense)
+    Bnk = rng.integers(-2, 3, (48, nb * 16)).astype(np.float64)
+    Cs = K.bsr_spmm(A, torch.from_numpy(Bnk).to(dev), precision=K.PREC_F64).cpu().numpy()
+    np.testing.assert_array_equal(Cs, dense @ Bnk.T)
