@@ -1,9 +1,9 @@
 // Block-GEMM instances: FP16 inputs, FP32 accumulation (16x16x32 and 16x16x16 f16 MFMA).
 // Shapes are the reference's FP16 sweep (scripts/fp16_block_square_GH200.sh:11-25,
-// scripts/fp16_block_lowrank_GH200.sh:20-27) restricted to the square sizes
-// {32, 64, 128, 256} and the low-rank 256 x 256 x {16, 32}.  A 3D partition at
-// M = N = 256 is not offered: its per-wave FP32 accumulator tile exceeds the register
-// file at every legal workgroup size.
+// scripts/fp16_block_lowrank_GH200.sh:20-27): squares {16, 32, 64, 128, 192, 256} and
+// low-rank M = N in {32, 64, 128, 192, 256} with K in {16, 32}.  A 3D partition at
+// M = N >= 192 is not offered: its per-wave FP32 accumulator tile exceeds the register
+// file at every legal workgroup size; low-rank K = 16 has no 3D split of K either.
 #include "block_gemm_registry.hpp"
 
 namespace kami {
@@ -46,6 +46,41 @@ void register_f16_instances(std::vector<BlockInstance>& v) {
     add_instance<A, F, 256, 256, 32, 16, 1, 1>(v, 1, 16);
     add_instance<A, F, 256, 256, 32, 4, 4, 1>(v, 2, 4);
     // persistent 2D forms (pipeline 2)
+    // 16^3 (one wave) and 192^3 (the largest size of the reference's FP16 square sweep,
+    // scripts/fp16_block_square_GH200.sh:13-26; 2D only: 1D and 3D tiles at 192 spill)
+    add_instance<H, F, 16, 16, 16, 1, 1, 1>(v, 1, 1);
+    add_instance<H, F, 16, 16, 16, 1, 1, 1>(v, 2, 1);
+    add_instance<H, F, 16, 16, 16, 1, 1, 1>(v, 3, 1);
+    add_instance<A, F, 192, 192, 192, 4, 4, 1>(v, 2, 4);
+    // low-rank M = N in {32, 64, 128, 192}, K in {16, 32}
+    // (scripts/fp16_block_lowrank_GH200.sh:20-27)
+    add_instance<H, F, 32, 32, 16, 1, 1, 1>(v, 1, 1);
+    add_instance<H, F, 32, 32, 16, 2, 1, 1>(v, 1, 2);
+    add_instance<H, F, 32, 32, 16, 1, 1, 1>(v, 2, 1);
+    add_instance<H, F, 32, 32, 16, 2, 2, 1>(v, 2, 2);
+    add_instance<H, F, 64, 64, 16, 1, 1, 1>(v, 1, 1);
+    add_instance<H, F, 64, 64, 16, 2, 1, 1>(v, 1, 2);
+    add_instance<H, F, 64, 64, 16, 4, 1, 1>(v, 1, 4);
+    add_instance<H, F, 64, 64, 16, 2, 2, 1>(v, 2, 2);
+    add_instance<A, F, 64, 64, 32, 1, 1, 1>(v, 1, 1);
+    add_instance<A, F, 64, 64, 32, 2, 1, 1>(v, 1, 2);
+    add_instance<A, F, 64, 64, 32, 4, 1, 1>(v, 1, 4);
+    add_instance<A, F, 64, 64, 32, 2, 2, 1>(v, 2, 2);
+    add_instance<H, F, 128, 128, 16, 2, 1, 1>(v, 1, 2);
+    add_instance<H, F, 128, 128, 16, 4, 1, 1>(v, 1, 4);
+    add_instance<H, F, 128, 128, 16, 8, 1, 1>(v, 1, 8);
+    add_instance<H, F, 128, 128, 16, 2, 2, 1>(v, 2, 2);
+    add_instance<A, F, 128, 128, 32, 2, 1, 1>(v, 1, 2);
+    add_instance<A, F, 128, 128, 32, 4, 1, 1>(v, 1, 4);
+    add_instance<A, F, 128, 128, 32, 8, 1, 1>(v, 1, 8);
+    add_instance<A, F, 128, 128, 32, 2, 2, 1>(v, 2, 2);
+    add_instance<H, F, 192, 192, 16, 4, 1, 1>(v, 1, 4);
+    add_instance<H, F, 192, 192, 16, 12, 1, 1>(v, 1, 12);
+    add_instance<H, F, 192, 192, 16, 4, 4, 1>(v, 2, 4);
+    add_instance<A, F, 192, 192, 32, 4, 1, 1>(v, 1, 4);
+    add_instance<A, F, 192, 192, 32, 12, 1, 1>(v, 1, 12);
+    add_instance<A, F, 192, 192, 32, 4, 4, 1>(v, 2, 4);
+
     add_pipe_instance<A, F, 64, 64, 64, 2, 2, 4>(v, 2);
     add_pipe_instance<A, F, 64, 64, 64, 4, 4, 4>(v, 4);
     add_pipe_instance<A, F, 128, 128, 128, 2, 2, 4>(v, 2);

@@ -227,13 +227,24 @@ const BlockInstance* resolve_block(const kami_gemm_desc* d, kami_status* st) {
         }
         pipe = 1;
     }
-    if (ranks == 0) ranks = default_ranks(d->precision, part, d->m, d->n, d->k);
+    const bool ranks_defaulted = ranks == 0;
+    if (ranks_defaulted) ranks = default_ranks(d->precision, part, d->m, d->n, d->k);
     for (const auto& in : registry()) {
         if (in.prec == d->precision && in.m == d->m && in.n == d->n && in.k == d->k &&
             in.part == part && in.ranks == ranks && in.pipe == pipe) {
             *st = KAMI_OK;
             return &in;
         }
+    }
+    // a defaulted wave count that has no compiled instance of this shape: the first
+    // compiled instance of the requested partition
+    if (ranks_defaulted) {
+        for (const auto& in : registry())
+            if (in.prec == d->precision && in.m == d->m && in.n == d->n && in.k == d->k &&
+                in.part == part && in.pipe == pipe) {
+                *st = KAMI_OK;
+                return &in;
+            }
     }
     // AUTO falls back to any compiled per-problem instance of the shape
     if (d->partition == KAMI_PART_AUTO) {

@@ -19,16 +19,16 @@ pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
 
-SQUARE = {K.PREC_F64: [16, 32, 64, 128], K.PREC_F16_F32: [32, 64, 128, 256],
-          K.PREC_BF16_F32: [32, 64, 128, 256]}
-LOWRANK = [(256, 256, 16), (256, 256, 32)]
+SQUARE = {K.PREC_F64: [16, 32, 64, 128], K.PREC_F16_F32: [16, 32, 64, 128, 192, 256],
+          K.PREC_BF16_F32: [16, 32, 64, 128, 192, 256]}
+LOWRANK = [(s, s, k) for s in (32, 64, 128, 192, 256) for k in (16, 32) if s != k]
 
 
 def instances(prec, shapes):
     out = []
     for (m, n, k) in shapes:
         for part in ("1d", "2d", "3d"):
-            for r in (1, 2, 4, 8, 16):
+            for r in (1, 2, 4, 8, 12, 16):
                 try:
                     K.query(m, n, k, prec, partition=part, ranks=r)
                 except K.KamiError:
