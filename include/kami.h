@@ -168,6 +168,17 @@ kami_status kami_block_gemm_autotune(const kami_gemm_desc* d, const void* A, con
                                      void* C, int64_t count, void* stream,
                                      kami_gemm_config* chosen);
 
+/* FP64 GEMM through FP16 / BF16 slices at sizes beyond the block instances: same
+ * arguments as kami_block_gemm with precision KAMI_PREC_F64_SPLIT_F16 or _BF16, B stored
+ * N x K, iters = 1, partition AUTO or 2D; M and N multiples of 128 (or of 64), K a
+ * multiple of 32.  Each problem is cut into 128 x 128 (64 x 64) tiles of C; persistent
+ * workgroups walk the tiles of all problems.  Error bound as for the block form, with
+ * digit exponents per row and 32-wide chunk of K.  No counterpart in the reference (it
+ * has no split-precision path); it widens the split-precision block GEMM to any size. */
+kami_status kami_split_gemm(const kami_gemm_desc* d, const void* A, const void* B, void* C,
+                            int64_t count, int64_t stride_a, int64_t stride_b,
+                            int64_t stride_c, void* stream);
+
 /* ---------------------------------------------------------------- batched GEMM */
 /* precision: KAMI_PREC_F64 or KAMI_PREC_F16_F32 or KAMI_PREC_BF16_F32.
  * Builds the work queue, autotunes per size bucket (runs kernels on `stream` when

@@ -119,6 +119,37 @@ def block_gemm(A, B, C=None, *, precision=None, partition="auto", ranks=0, b_lay
     return C[0] if squeeze else C
 
 
+def split_gemm(A, B, C=None, *, precision=PREC_F64_SPLIT_F16, slices=0, alpha=1.0, beta=0.0,
+               stream=None):
+    """FP64 C = alpha * A @ B^T + beta * C through FP16/BF16 slices at any size whose M and
+    N are multiples of 128 (or 64) and K a multiple of 32 (kami_split_gemm).
+
+    A: (count, M, K) or (M, K) float64; B: (count, N, K) or (N, K); C like A @ B^T.
+    """
+    import torch
+    squeeze = A.dim() == 2
+    if squeeze:
+        A, B = A.unsqueeze(0), B.unsqueeze(0)
+        if C is not None:
+            C = C.unsqueeze(0)
+    count, M, K = A.shape
+    N = B.shape[1]
+    if C is None:
+        C = torch.empty((count, M, N), dtype=torch.float64, device=A.device)
+    for t in (A, B, C):
+        if not t.is_contiguous() or t.dtype != torch.float64:
+            raise ValueError("split_gemm expects contiguous float64 tensors")
+    d = make_desc(M, N, K, precision, alpha=alpha, beta=beta, slices=slices)
+    check(lib().kami_split_gemm(ctypes.byref(d), ctypes.c_void_p(A.data_ptr()),
+                                ctypes.c_void_p(B.data_ptr()), ctypes.c_void_p(C.data_ptr()),
+                                count, 0, 0, 0, _stream_ptr(stream)),
+          f"kami_split_gemm({M}x{N}x{K}, prec={precision})")
+    return C[0] if squeeze else C
+
+
+__all__ += ["split_gemm"]
+
+
 class BatchedPlan:
     """A batched GEMM plan over problems of uneven size (kami_batched_plan_*).
 

@@ -79,6 +79,8 @@ _PROTOTYPES = {
     "kami_block_gemm_query": (c_int32, [ctypes.POINTER(GemmDesc), ctypes.POINTER(GemmConfig)]),
     "kami_block_gemm": (c_int32, [ctypes.POINTER(GemmDesc), c_void_p, c_void_p, c_void_p,
                                   c_int64, c_int64, c_int64, c_int64, c_void_p]),
+    "kami_split_gemm": (c_int32, [ctypes.POINTER(GemmDesc), c_void_p, c_void_p, c_void_p,
+                                  c_int64, c_int64, c_int64, c_int64, c_void_p]),
     "kami_block_gemm_autotune": (c_int32, [ctypes.POINTER(GemmDesc), c_void_p, c_void_p, c_void_p,
                                            c_int64, c_void_p, ctypes.POINTER(GemmConfig)]),
     "kami_batched_plan_create": (c_int32, [ctypes.POINTER(Problem), c_int64, c_int32,

@@ -323,9 +323,11 @@ kami_status kami_block_gemm_query(const kami_gemm_desc* d, kami_gemm_config* out
     return KAMI_OK;
 }
 
-kami_status kami_block_gemm(const kami_gemm_desc* d, const void* A, const void* B, void* C,
-                            int64_t count, int64_t stride_a, int64_t stride_b, int64_t stride_c,
-                            void* stream) {
+// Operand pointers, leading dimensions and problem strides of a kami_block_gemm-style
+// call (strides of 0 mean packed problems).
+static kami_status prepare_args(const kami_gemm_desc* d, const void* A, const void* B, void* C,
+                                int64_t count, int64_t stride_a, int64_t stride_b,
+                                int64_t stride_c, BlockArgs& args) {
     kami_status st = validate_desc(d);
     if (st != KAMI_OK) return st;
     if (count < 0 || count > 0x7fffffffLL) return KAMI_EINVAL;
@@ -334,7 +336,6 @@ kami_status kami_block_gemm(const kami_gemm_desc* d, const void* A, const void* 
     if (!A || !B || !C) return KAMI_EINVAL;
     const int ei = elt_in(d->precision), eo = elt_out(d->precision);
     if (((uintptr_t)A % 16) || ((uintptr_t)B % 16) || ((uintptr_t)C % eo)) return KAMI_EINVAL;
-    BlockArgs args;
     args.A = A;
     args.B = B;
     args.C = C;
@@ -353,6 +354,15 @@ kami_status kami_block_gemm(const kami_gemm_desc* d, const void* A, const void* 
     args.beta = d->beta;
     args.iters = d->iters;
     args.b_kn = d->b_layout == KAMI_B_KN;
+    return KAMI_OK;
+}
+
+kami_status kami_block_gemm(const kami_gemm_desc* d, const void* A, const void* B, void* C,
+                            int64_t count, int64_t stride_a, int64_t stride_b, int64_t stride_c,
+                            void* stream) {
+    BlockArgs args;
+    kami_status st = prepare_args(d, A, B, C, count, stride_a, stride_b, stride_c, args);
+    if (st != KAMI_OK || count == 0) return st;
     if (is_split(d->precision)) {
         st = device_check();
         if (st != KAMI_OK) return st;
@@ -365,6 +375,15 @@ kami_status kami_block_gemm(const kami_gemm_desc* d, const void* A, const void* 
     hipError_t e = in->launch(args, count, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e);
     return KAMI_OK;
+}
+
+kami_status kami_split_gemm(const kami_gemm_desc* d, const void* A, const void* B, void* C,
+                            int64_t count, int64_t stride_a, int64_t stride_b, int64_t stride_c,
+                            void* stream) {
+    BlockArgs args;
+    kami_status st = prepare_args(d, A, B, C, count, stride_a, stride_b, stride_c, args);
+    if (st != KAMI_OK || count == 0) return st;
+    return split_gemm_tiled(d, args, count, (hipStream_t)stream);
 }
 
 kami_status kami_block_gemm_autotune(const kami_gemm_desc* d, const void* A, const void* B,

@@ -39,5 +39,7 @@ inline bool is_split(int prec) {
 kami_status split_query(const kami_gemm_desc* d, kami_gemm_config* out);
 kami_status split_launch(const kami_gemm_desc* d, const BlockArgs& args, int64_t count,
                          hipStream_t stream);
+kami_status split_gemm_tiled(const kami_gemm_desc* d, const BlockArgs& args, int64_t count,
+                             hipStream_t stream);
 
 }  // namespace kami

@@ -23,6 +23,7 @@
 // The error relative to 2^(e_i + f_j) is about K 2^(-BETA S - 1) from the truncated
 // digits plus the dropped levels g > G; finite inputs are required.
 #include <cmath>
+#include <cstdint>
 
 #include "kami_internal.hpp"
 
@@ -93,6 +94,9 @@ struct SplitArgs {
     double* C;
     int64_t sa, sb, sc, lda, ldb, ldc;
     double alpha, beta;
+    // persistent form only: K chunks per problem and the grid of BM x BN tiles of C per
+    // problem (1 x 1 for a block problem)
+    int32_t nch, tiles_m, tiles_n;
 };
 
 __device__ inline int exp_of(double x) {
@@ -310,8 +314,10 @@ split_gemm_kernel(SplitArgs args) {
         }
 }
 
-// Persistent form.  A workgroup walks problems blockIdx.x, + gridDim.x, ... and reads
-// every operand once: the exponents are taken per row and per 32-wide K chunk (so no
+// Persistent form.  A workgroup walks work units blockIdx.x, + gridDim.x, ... -- one unit
+// is a BM x BN tile of C of one problem (the whole problem for a block GEMM; a tile of a
+// larger GEMM for kami_split_gemm, with K any multiple of 32) -- and reads every operand
+// once: the exponents are taken per row and per 32-wide K chunk (so no
 // separate exponent pass over K is needed), the chunk is sliced from registers, and the
 // next chunk -- chunk 0 of the next problem after the last one -- is loaded while the
 // MFMAs of the current chunk run.  A chunk's level sums are combined in FP64 with the
@@ -331,13 +337,27 @@ split_pipe_kernel(SplitArgs args, int64_t count) {
     const int pn = wave % G::PN, pm = wave / G::PN;
     const int64_t grid = gridDim.x;
     const int64_t lda = args.lda, ldb = args.ldb, ldc = args.ldc;
+    const int nch = args.nch;
+    const int64_t tiles = (int64_t)args.tiles_m * args.tiles_n;
+    // work unit u = one BM x BN tile of one problem; tiles of a problem are consecutive
+    // with the column tile fastest, so neighbouring workgroups share A row panels
+    auto unit_base = [&](int64_t u, int64_t& a_off, int64_t& b_off, int64_t& c_off) {
+        const int64_t prob = u / tiles;
+        const int t = (int)(u - prob * tiles);
+        const int tm = t / args.tiles_n, tn = t - tm * args.tiles_n;
+        a_off = prob * args.sa + (int64_t)tm * G::BM * lda;
+        b_off = prob * args.sb + (int64_t)tn * G::BN * ldb;
+        c_off = prob * args.sc + (int64_t)tm * G::BM * ldc + (int64_t)tn * G::BN;
+    };
 
     // raw pieces of one chunk: thread t owns 16-byte pieces q = t + i * NT (row q / 16,
     // k pair q % 16), 16 consecutive lanes per row
     u4_t xa[G::PA], xb[G::PB];
-    auto load_chunk = [&](int64_t p, int c) {
-        const double* __restrict__ A = args.A + p * args.sa;
-        const double* __restrict__ B = args.B + p * args.sb;
+    auto load_chunk = [&](int64_t u, int c) {
+        int64_t a_off, b_off, c_off;
+        unit_base(u, a_off, b_off, c_off);
+        const double* __restrict__ A = args.A + a_off;
+        const double* __restrict__ B = args.B + b_off;
 #pragma unroll
         for (int i = 0; i < G::PA; ++i) {
             const int q = tid + i * G::NT;
@@ -370,7 +390,7 @@ split_pipe_kernel(SplitArgs args, int64_t count) {
         }
     };
 
-    int64_t p = blockIdx.x;
+    int64_t p = blockIdx.x;  // work unit
     if (p >= count) return;  // uniform
     load_chunk(p, 0);
 
@@ -383,7 +403,7 @@ split_pipe_kernel(SplitArgs args, int64_t count) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) acc[i][j][v] = 0.0;
 
-        for (int c = 0; c < G::NCH; ++c) {
+        for (int c = 0; c < nch; ++c) {
             // ---- exponents and digit planes of this chunk ----
 #pragma unroll
             for (int i = 0; i < G::PA; ++i) {
@@ -400,7 +420,7 @@ split_pipe_kernel(SplitArgs args, int64_t count) {
                 slice_piece(xb[i], row, kp, e, smem + G::SA_BYTES);
             }
             // the next chunk is in flight during the products
-            if (c + 1 < G::NCH)
+            if (c + 1 < nch)
                 load_chunk(p, c + 1);
             else if (p + grid < count)
                 load_chunk(p + grid, 0);
@@ -453,7 +473,9 @@ split_pipe_kernel(SplitArgs args, int64_t count) {
         }
 
         // ---- store ----
-        double* __restrict__ C = args.C + p * args.sc;
+        int64_t a_off, b_off, c_off;
+        unit_base(p, a_off, b_off, c_off);
+        double* __restrict__ C = args.C + c_off;
 #pragma unroll
         for (int im = 0; im < G::IM; ++im)
 #pragma unroll
@@ -580,7 +602,50 @@ kami_status split_launch(const kami_gemm_desc* d, const BlockArgs& b, int64_t co
     a.ldc = b.ldc;
     a.alpha = b.alpha;
     a.beta = b.beta;
+    a.nch = d->k / 32;
+    a.tiles_m = 1;
+    a.tiles_n = 1;
     hipError_t e = in->launch(a, count, stream);
+    if (e != hipSuccess) return hip_fail(e);
+    return KAMI_OK;
+}
+
+// Any M, N that are multiples of the tile edge (128, else 64) and K a multiple of 32:
+// every problem is cut into tiles of C, each a work unit of the persistent kernel.
+kami_status split_gemm_tiled(const kami_gemm_desc* d, const BlockArgs& b, int64_t count,
+                             hipStream_t stream) {
+    if (!is_split(d->precision)) return KAMI_ENOTSUP;
+    if (d->b_layout != KAMI_B_NK || d->iters != 1) return KAMI_ENOTSUP;
+    if (d->partition != KAMI_PART_AUTO && d->partition != KAMI_PART_2D) return KAMI_ENOTSUP;
+    if (d->k % 32 != 0) return KAMI_ENOTSUP;
+    const int tile = (d->m % 128 == 0 && d->n % 128 == 0) ? 128
+                     : (d->m % 64 == 0 && d->n % 64 == 0) ? 64 : 0;
+    if (!tile) return KAMI_ENOTSUP;
+    kami_gemm_desc t = *d;
+    t.m = t.n = t.k = tile;
+    t.pipeline = 2;
+    const SplitInstance* in = resolve_split(&t);
+    if (!in) return KAMI_ENOTSUP;
+    const int64_t tiles = (int64_t)(d->m / tile) * (d->n / tile);
+    if (count > INT64_MAX / tiles) return KAMI_EINVAL;
+    SplitArgs a;
+    a.A = reinterpret_cast<const double*>(b.A);
+    a.B = reinterpret_cast<const double*>(b.B);
+    a.C = reinterpret_cast<double*>(b.C);
+    a.sa = b.sa;
+    a.sb = b.sb;
+    a.sc = b.sc;
+    a.lda = b.lda;
+    a.ldb = b.ldb;
+    a.ldc = b.ldc;
+    a.alpha = b.alpha;
+    a.beta = b.beta;
+    a.nch = d->k / 32;
+    a.tiles_m = d->m / tile;
+    a.tiles_n = d->n / tile;
+    kami_status st = device_check();
+    if (st != KAMI_OK) return st;
+    hipError_t e = in->launch(a, count * tiles, stream);
     if (e != hipSuccess) return hip_fail(e);
     return KAMI_OK;
 }

@@ -165,3 +165,21 @@ def test_spmm_validation_without_device():
     assert call() == _abi.ENOTSUP
     A.bs = 16
     assert call() == _abi.ENODEV              # valid request, no GPU here
+
+
+def test_split_gemm_validation_without_device():
+    lib = K.lib()
+
+    def call(m, n, k, prec=K.PREC_F64_SPLIT_F16, **kw):
+        d = K.make_desc(m, n, k, prec, **kw)
+        return lib.kami_split_gemm(ctypes.byref(d), 1 << 20, 1 << 21, 1 << 22, 1, 0, 0, 0, None)
+
+    assert call(256, 256, 256, K.PREC_F64) == _abi.ENOTSUP       # not a split precision
+    assert call(96, 128, 64) == _abi.ENOTSUP                      # M not a multiple of 64
+    assert call(128, 128, 48) == _abi.ENOTSUP                     # K not a multiple of 32
+    assert call(128, 128, 64, b_layout=K.B_KN) == _abi.ENOTSUP
+    assert call(128, 128, 64, partition="3d") == _abi.ENOTSUP
+    assert call(128, 128, 64, slices=3) == _abi.ENOTSUP           # no such instance
+    assert call(0, 128, 64) == _abi.EINVAL
+    for shape in [(256, 384, 96), (192, 64, 32), (4096, 4096, 4096)]:
+        assert call(*shape) == _abi.ENODEV                        # valid; no GPU here

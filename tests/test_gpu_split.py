@@ -156,3 +156,43 @@ def test_split_unsupported_requests_are_enotsup():
         assert e.value.status == _abi.ENOTSUP
     with pytest.raises(K.KamiError):
         K.query(96, 96, 96, K.PREC_F64_SPLIT_F16)
+
+
+SPLIT_GEMM_SHAPES = [(256, 384, 96), (192, 320, 64), (128, 128, 1024), (64, 448, 32)]
+
+
+@pytest.mark.parametrize("prec,slices", SPLITS, ids=SIDS)
+def test_split_gemm_any_size_kat_exact(prec, slices, dev):
+    """kami_split_gemm cuts larger problems into tiles; integer inputs stay exact."""
+    for (m, n, k) in SPLIT_GEMM_SHAPES:
+        count = 2
+        A = oracle.randr_mod3(count * m * k).reshape(count, m, k) - 1.0
+        B = (oracle.randr_mod3(2 * count * n * k)[count * n * k:].reshape(count, n, k) - 1.0) * 7.0
+        try:
+            C = K.split_gemm(torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev),
+                             precision=prec, slices=slices)
+        except K.KamiError as e:
+            assert e.status == _abi.ENOTSUP  # no 64-tile instance for this slice count
+            continue
+        np.testing.assert_array_equal(C.cpu().numpy(), oracle.gemm_f64(A, B))
+
+
+@pytest.mark.parametrize("prec,slices", SPLITS, ids=SIDS)
+def test_split_gemm_log_uniform_normwise(prec, slices, dev):
+    rng = np.random.default_rng(0x4B414D49 + 8)
+    for (m, n, k) in SPLIT_GEMM_SHAPES[:3]:
+        A = log_uniform(rng, (1, m, k))
+        B = log_uniform(rng, (1, n, k))
+        C0 = rng.standard_normal((1, m, n))
+        try:
+            Cd = torch.from_numpy(C0).to(dev)
+            K.split_gemm(torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev), Cd,
+                         precision=prec, slices=slices, alpha=0.5, beta=-2.0)
+        except K.KamiError as e:
+            assert e.status == _abi.ENOTSUP
+            continue
+        exact, _ = oracle.gemm_dot2(A, B)
+        ref = 0.5 * exact - 2.0 * C0
+        err = normwise(Cd.cpu().numpy(), ref)
+        print(f"  {m}x{n}x{k} normwise {err:.3e}", flush=True)
+        assert err <= 1e-13
