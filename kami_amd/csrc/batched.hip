@@ -25,9 +25,20 @@
 
 namespace kami {
 
+// One queue entry = one output tile, carrying everything its loads and stores need, so
+// the kernel never goes back to the problem table.
 struct WorkItem {
-    int32_t prob, row0, col0, variant;
+    const void* A;  // first element of the tile's rows of A
+    const void* B;  // first element of the tile's rows of B (n x K) or columns (K x n)
+    void* C;        // first element of the tile in C
+    int64_t lda, ldb, ldc;
+    int32_t mlim, nlim;  // rows / columns of C from the tile origin to the problem edge
+    int32_t k, variant;
 };
+static_assert(sizeof(WorkItem) == 64, "queue entry layout");
+
+// Queue entries popped per atomic add.
+constexpr int kQueueBatch = 8;
 
 // Staging shared by every tile variant: a stage holds up to TMAX rows of A and TNMAX rows
 // of B (or KC rows of TNMAX columns when B is K x N), KC elements of K each, in LDS rows
@@ -36,8 +47,11 @@ template <class AT>
 struct StreamGeom {
     static constexpr int NT = 256;
     static constexpr int ELT = (int)sizeof(typename AT::in_t), VE = 16 / ELT;
+    // FP64 stages 32-wide K chunks of tiles up to 64 x 64 (the 128-row variant is not
+    // offered), FP16/BF16 64-wide chunks of tiles up to 128 x 64: about 64 KB / 55 KB of
+    // LDS for the two stage buffers, two workgroups per CU
     static constexpr int KC = ELT == 8 ? 16 : 64;
-    static constexpr int TMAX = 128, TNMAX = 64;
+    static constexpr int TMAX = ELT == 8 ? 64 : 128, TNMAX = 64;
     static constexpr int ROWB = KC * ELT + 16;
     static constexpr int A_REGION = TMAX * ROWB;
     static constexpr int STAGE = (TMAX + TNMAX) * ROWB;
@@ -45,6 +59,7 @@ struct StreamGeom {
     static constexpr int CPR = KC / VE;  // pieces per staged row
     static constexpr int NP = ((TMAX + TNMAX) * CPR + NT - 1) / NT;
     static_assert((KC * TNMAX) / VE <= TNMAX * CPR, "K x N staging fits the B region");
+    static_assert((TMAX * CPR) % NT == 0 && (TNMAX * CPR) % NT == 0, "whole piece groups");
 };
 
 // Tile geometry of a variant: TM x TN output tile, WM x WN waves (4 waves).
@@ -73,11 +88,13 @@ struct Variants {
     using V3 = TileGeom<AT, 128, 64, 2, 2>;
 };
 
+// Variants a precision offers: FP64 stages at most 64 rows, so not variant 3.
+inline int variants_of(int prec) { return prec == KAMI_PREC_F64 ? 3 : kVariants; }
+
 __device__ inline int variant_tm(int v) { return v == 1 ? 32 : (v == 3 ? 128 : 64); }
 __device__ inline int variant_tn(int v) { return v == 1 ? 32 : 64; }
 
 struct BatchArgs {
-    const kami_problem* probs;
     const WorkItem* items;
     unsigned int* counter;
     int64_t n_items;
@@ -85,67 +102,120 @@ struct BatchArgs {
     int32_t b_kn;
 };
 
-// A queue entry as the stream sees it (identical in every thread of a workgroup).
-struct TileRef {
-    int prob, row0, col0, variant, nch;
+// A tile as the kernel holds it: wave-uniform values (in scalar registers).
+struct Tile {
+    const unsigned char* A;
+    const unsigned char* B;
+    unsigned char* C;
+    int64_t lda, ldb, ldc;
+    int mlim, nlim, k, variant, nch;
     bool valid;
 };
 
-// Loads the pieces of stage (tile t, K chunk c) into registers; rows outside C and K
-// beyond the problem are zero.
-template <class AT>
-__device__ __forceinline__ void stream_load(const BatchArgs& a, const TileRef& t, int c, bool b_kn,
-                            u4_t (&r)[StreamGeom<AT>::NP]) {
-    using SG = StreamGeom<AT>;
-    using in_t = typename AT::in_t;
-    const kami_problem P = a.probs[t.prob];
-    const int tm = variant_tm(t.variant), tn = variant_tn(t.variant);
-    const int mlim = P.m - t.row0, nlim = P.n - t.col0, K = P.k, k0 = c * SG::KC;
-    const in_t* A = reinterpret_cast<const in_t*>(P.A);
-    const in_t* B = reinterpret_cast<const in_t*>(P.B);
-    const int ca = tm * SG::CPR;
-#pragma unroll
-    for (int i = 0; i < SG::NP; ++i) {
-        const int idx = threadIdx.x + i * SG::NT;
-        u4_t v = {0, 0, 0, 0};
-        if (idx < ca) {
-            const int row = idx / SG::CPR, kc = (idx % SG::CPR) * SG::VE;
-            if (row < mlim && k0 + kc < K) v = load_raw<16>(A + (int64_t)(t.row0 + row) * P.lda + k0 + kc);
-        } else if (!b_kn) {
-            const int j = idx - ca;
-            const int row = j / SG::CPR, kc = (j % SG::CPR) * SG::VE;
-            if (row < tn && row < nlim && k0 + kc < K)
-                v = load_raw<16>(B + (int64_t)(t.col0 + row) * P.ldb + k0 + kc);
-        } else {
-            const int j = idx - ca, ppr = tn / SG::VE;  // pieces per K row of the tile
-            const int kr = j / ppr, jc = (j % ppr) * SG::VE;
-            if (kr < SG::KC && jc < nlim && k0 + kr < K)
-                v = load_raw<16>(B + (int64_t)(k0 + kr) * P.ldb + t.col0 + jc);
-        }
-        r[i] = v;
-    }
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int64_t uni64(int64_t x) {
+    const uint64_t u = (uint64_t)x;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+template <class P>
+__device__ __forceinline__ P* uniptr(P* p) {
+    return reinterpret_cast<P*>(uni64((int64_t)reinterpret_cast<uintptr_t>(p)));
 }
 
 template <class AT>
-__device__ __forceinline__ void stream_write(int variant, bool b_kn, const u4_t (&r)[StreamGeom<AT>::NP],
-                             unsigned char* buf) {
+__device__ __forceinline__ Tile read_tile(const WorkItem* rec, bool valid) {
+    Tile t;
+    t.valid = valid;
+    if (!valid) {
+        t.A = t.B = t.C = nullptr;
+        t.lda = t.ldb = t.ldc = 0;
+        t.mlim = t.nlim = t.k = t.variant = t.nch = 0;
+        return t;
+    }
+    const WorkItem w = *rec;
+    t.A = uniptr(reinterpret_cast<const unsigned char*>(w.A));
+    t.B = uniptr(reinterpret_cast<const unsigned char*>(w.B));
+    t.C = uniptr(reinterpret_cast<unsigned char*>(w.C));
+    t.lda = uni64(w.lda);
+    t.ldb = uni64(w.ldb);
+    t.ldc = uni64(w.ldc);
+    t.mlim = uni(w.mlim);
+    t.nlim = uni(w.nlim);
+    t.k = uni(w.k);
+    t.variant = uni(w.variant);
+    t.nch = (t.k + StreamGeom<AT>::KC - 1) / StreamGeom<AT>::KC;
+    return t;
+}
+
+// Raw buffer descriptors over a tile's operands.  An offset at or past kBufferRange reads
+// as zero, so pieces outside C or past K need no branch.
+constexpr int kBufferRange = 0x7FFFFFF0;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const unsigned char* base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<unsigned char*>(base), (short)0,
+                                             kBufferRange, 0x00020000);
+}
+__device__ __forceinline__ u4_t buffer_load16(__amdgpu_buffer_rsrc_t r, int off) {
+    return __builtin_bit_cast(u4_t, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+
+// Loads the pieces of stage (tile t, K chunk c) into registers.  Every stage covers the
+// full staging area (TMAX rows of A, TNMAX rows of B, or KC rows of TNMAX columns when B
+// is K x N) with a fixed piece-to-thread mapping; rows outside the tile or C and K past
+// the problem read as zero (no memory traffic), so the code is the same for every tile
+// variant and free of branches.
+template <class AT, bool KN>
+__device__ __forceinline__ void stream_load(const Tile& t, int c, u4_t (&r)[StreamGeom<AT>::NP]) {
     using SG = StreamGeom<AT>;
-    using in_t = typename AT::in_t;
-    const int tm = variant_tm(variant), tn = variant_tn(variant);
-    const int ca = tm * SG::CPR;
+    constexpr int CA = SG::TMAX * SG::CPR, CB = SG::TNMAX * SG::CPR;
+    const int k0 = c * SG::KC;
+    const int mrows = min(variant_tm(t.variant), t.mlim);
+    const int ncols = min(variant_tn(t.variant), t.nlim);
+    const __amdgpu_buffer_rsrc_t ra = tile_rsrc(t.A), rb = tile_rsrc(t.B);
+    const int lda = (int)t.lda, ldb = (int)t.ldb;
 #pragma unroll
     for (int i = 0; i < SG::NP; ++i) {
         const int idx = threadIdx.x + i * SG::NT;
-        if (idx < ca) {
-            store_raw<16>(buf + (idx / SG::CPR) * SG::ROWB + (idx % SG::CPR) * 16, r[i]);
-        } else if (!b_kn) {
-            const int j = idx - ca;
-            if (j < tn * SG::CPR)
-                store_raw<16>(buf + SG::A_REGION + (j / SG::CPR) * SG::ROWB + (j % SG::CPR) * 16, r[i]);
+        if (i * SG::NT < CA) {  // compile-time: the group is all A
+            const int row = idx / SG::CPR, kc = (idx % SG::CPR) * SG::VE;
+            const bool ok = row < mrows && k0 + kc < t.k;
+            r[i] = buffer_load16(ra, ok ? (row * lda + k0 + kc) * SG::ELT : kBufferRange);
+        } else if (i * SG::NT < CA + CB) {  // all B
+            const int j = idx - CA;
+            int off;
+            if constexpr (!KN) {
+                const int row = j / SG::CPR, kc = (j % SG::CPR) * SG::VE;
+                off = (row < ncols && k0 + kc < t.k) ? (row * ldb + k0 + kc) * SG::ELT : kBufferRange;
+            } else {
+                constexpr int PPR = SG::TNMAX / SG::VE;  // pieces per K row
+                const int kr = j / PPR, jc = (j % PPR) * SG::VE;
+                off = (jc < ncols && k0 + kr < t.k) ? ((k0 + kr) * ldb + jc) * SG::ELT : kBufferRange;
+            }
+            r[i] = buffer_load16(rb, off);
         } else {
-            const int j = idx - ca, ppr = tn / SG::VE;
-            const int kr = j / ppr, jc = (j % ppr) * SG::VE;
-            if (kr < SG::KC) {
+            r[i] = u4_t{0, 0, 0, 0};
+        }
+    }
+}
+
+template <class AT, bool KN>
+__device__ __forceinline__ void stream_write(const u4_t (&r)[StreamGeom<AT>::NP], unsigned char* buf) {
+    using SG = StreamGeom<AT>;
+    using in_t = typename AT::in_t;
+    constexpr int CA = SG::TMAX * SG::CPR, CB = SG::TNMAX * SG::CPR;
+#pragma unroll
+    for (int i = 0; i < SG::NP; ++i) {
+        const int idx = threadIdx.x + i * SG::NT;
+        if (i * SG::NT < CA) {
+            store_raw<16>(buf + (idx / SG::CPR) * SG::ROWB + (idx % SG::CPR) * 16, r[i]);
+        } else if (i * SG::NT < CA + CB) {
+            const int j = idx - CA;
+            if constexpr (!KN) {
+                store_raw<16>(buf + SG::A_REGION + (j / SG::CPR) * SG::ROWB + (j % SG::CPR) * 16, r[i]);
+            } else {
+                constexpr int PPR = SG::TNMAX / SG::VE;
+                const int kr = j / PPR, jc = (j % PPR) * SG::VE;
                 const in_t* e = reinterpret_cast<const in_t*>(&r[i]);
 #pragma unroll
                 for (int q = 0; q < SG::VE; ++q)
@@ -155,86 +225,159 @@ __device__ __forceinline__ void stream_write(int variant, bool b_kn, const u4_t 
     }
 }
 
-// Stream state of a workgroup: the current tile and two queued behind it, the position
-// of the next stage to load, and a two-deep register ring of staged pieces (slot 0 =
-// the stage after the one in LDS, slot 1 = the stage after that).
+// Shared state of a workgroup's stream of (tile, K chunk) stages.
+//   * The queue is popped kQueueBatch entries at a time into one of two LDS record
+//     buffers; stream tile s lives in buffer (s / Q) % 2, slot s % Q.  The next batch
+//     is fetched in steps taken at tile boundaries (atomic add; entry loads; LDS write),
+//     so its latency hides behind whole tiles.
+//   * Stage x of the stream (x counts (tile, chunk) pairs) is loaded into register slot
+//     x % 2 and computed from LDS buffer x % 2.  While stage x is computed, slot
+//     (x + 1) % 2 holds stage x + 1 and slot x % 2 holds stage x + 2.  The stage loop is
+//     unrolled in pairs so both slot indices are compile-time, and it issues no memory
+//     operation other than the stage loads: the compiler then waits only for the slot
+//     it writes, never for the loads still in flight.
+//   * The cursor (tile tc, chunk cc) is the next stage to load.  It stops ("blocked")
+//     at a tile whose record has not been fetched yet; tile boundaries unblock it and
+//     load any stage that is missing.
 template <class AT>
 struct Stream {
-    TileRef T[3];
-    int lt, lc;  // load cursor: tile index into T, K chunk
-    u4_t ring0[StreamGeom<AT>::NP], ring1[StreamGeom<AT>::NP];
-    int v0, v1;  // variant of the stage in each slot, -1 = empty
-    int buf;     // LDS buffer holding the stage being computed
+    Tile cur;            // tile being computed (stream index t0)
+    Tile ct;             // tile at the cursor (stream index tc)
+    int64_t t0, tc;
+    int cc;              // chunk of the cursor
+    bool blocked;
+    int64_t x;           // stage being computed
+    int64_t loaded;      // stages loaded so far
+    int64_t fetched;     // stream tiles whose records are in LDS (whole batches)
+    int phase;           // next batch fetch: 0 idle, 1 atomic issued, 2 entries loaded
+    u4_t slot0[StreamGeom<AT>::NP], slot1[StreamGeom<AT>::NP];
+    int sv0, sv1;        // variant of the stage in each slot, -1 = empty
+    // wave 0 only: batch fetch in flight
+    unsigned fbase;
+    u4_t fpiece;         // lane l < 4 Q: 16-byte piece l % 4 of entry l / 4
+    bool fvalid;
 };
 
-__device__ __forceinline__ TileRef fetch_tile(const BatchArgs& a, unsigned int* s_item) {
-    if (threadIdx.x == 0) *s_item = atomicAdd(a.counter, 1u);
-    __syncthreads();
-    const int64_t it = *s_item;
-    __syncthreads();
-    TileRef t;
-    t.valid = it < a.n_items;  // identical in every thread: the exit is uniform
-    if (t.valid) {
-        const WorkItem w = a.items[it];
-        t.prob = w.prob;
-        t.row0 = w.row0;
-        t.col0 = w.col0;
-        t.variant = w.variant;
-        t.nch = 0;
+struct QueueLds {
+    alignas(16) WorkItem rec[2][kQueueBatch];
+    int valid[2][kQueueBatch];
+};
+
+template <class AT>
+__device__ __forceinline__ Tile stream_tile(const QueueLds& q, int64_t s) {
+    const int b = (int)((s / kQueueBatch) & 1), slot = (int)(s % kQueueBatch);
+    return read_tile<AT>(&q.rec[b][slot], uni(q.valid[b][slot]) != 0);
+}
+
+// One step of the batch fetch (every thread, uniform); the caller puts a barrier after
+// it and then calls fetch_commit.
+template <class AT>
+__device__ __forceinline__ void fetch_step(const BatchArgs& a, Stream<AT>& st, QueueLds& q) {
+    st.phase = uni(st.phase);
+    if (st.phase == 0) {
+        // pop the queue range of stream tiles [fetched, fetched + Q)
+        if (threadIdx.x == 0) st.fbase = atomicAdd(a.counter, (unsigned)kQueueBatch);
+        st.phase = 1;
+    } else if (st.phase == 1) {
+        if (threadIdx.x < 64) {
+            const unsigned base = __shfl(st.fbase, 0);
+            const int lane = threadIdx.x, e = lane >> 2;
+            st.fvalid = e < kQueueBatch && (int64_t)base + e < a.n_items;
+            if (st.fvalid)
+                st.fpiece = load_raw<16>(reinterpret_cast<const unsigned char*>(a.items + base + e) +
+                                         16 * (lane & 3));
+        }
+        st.phase = 2;
     } else {
-        t.prob = t.row0 = t.col0 = t.variant = t.nch = 0;
+        const int b = (int)((st.fetched / kQueueBatch) & 1);
+        if (threadIdx.x < 4 * kQueueBatch) {
+            const int e = threadIdx.x >> 2;
+            if (st.fvalid)
+                store_raw<16>(reinterpret_cast<unsigned char*>(&q.rec[b][e]) + 16 * (threadIdx.x & 3),
+                              st.fpiece);
+            if ((threadIdx.x & 3) == 0) q.valid[b][e] = st.fvalid ? 1 : 0;
+        }
+        st.phase = 3;
     }
-    return t;
 }
 
 template <class AT>
-__device__ __forceinline__ void set_chunks(const BatchArgs& a, TileRef& t) {
-    if (t.valid) t.nch = (a.probs[t.prob].k + StreamGeom<AT>::KC - 1) / StreamGeom<AT>::KC;
+__device__ __forceinline__ void fetch_commit(Stream<AT>& st) {
+    if (st.phase == 3) {
+        st.fetched += kQueueBatch;
+        st.phase = -1;  // idle until the next batch is started
+    }
 }
 
-// Field-wise selection: a runtime index into the private array, or a select between
-// their addresses, would keep the whole stream state in scratch memory.
-__device__ __forceinline__ TileRef pick_tile(int i, const TileRef& x, const TileRef& y,
-                                             const TileRef& z) {
-    TileRef t;
-    t.prob = i == 0 ? x.prob : (i == 1 ? y.prob : z.prob);
-    t.row0 = i == 0 ? x.row0 : (i == 1 ? y.row0 : z.row0);
-    t.col0 = i == 0 ? x.col0 : (i == 1 ? y.col0 : z.col0);
-    t.variant = i == 0 ? x.variant : (i == 1 ? y.variant : z.variant);
-    t.nch = i == 0 ? x.nch : (i == 1 ? y.nch : z.nch);
-    t.valid = i == 0 ? x.valid : (i == 1 ? y.valid : z.valid);
-    return t;
-}
-
-// Load the stage at the cursor into `dst` (if any) and advance the cursor.
+// Makes the record of stream tile s available (uniform; synchronous).  The batch
+// fetched here goes to the LDS buffer of the batch two before it, which the current
+// tile and the cursor have both left.
 template <class AT>
-__device__ __forceinline__ int load_at_cursor(const BatchArgs& a, Stream<AT>& st, bool b_kn,
-                              u4_t (&dst)[StreamGeom<AT>::NP]) {
-    if (st.lt > 2) return -1;
-    const TileRef t = pick_tile(st.lt, st.T[0], st.T[1], st.T[2]);
-    if (!t.valid) return -1;
-    stream_load<AT>(a, t, st.lc, b_kn, dst);
-    const int v = t.variant;
-    if (++st.lc >= t.nch) {
-        st.lc = 0;
-        ++st.lt;
+__device__ __forceinline__ void ensure_ready(const BatchArgs& a, Stream<AT>& st, QueueLds& q, int64_t s) {
+    while (st.fetched <= s) {
+        if (st.phase < 0) st.phase = 0;
+        fetch_step(a, st, q);
+        __syncthreads();
+        fetch_commit(st);
+    }
+}
+
+// Load the stage at the cursor into `dst` and advance the cursor.  Returns the stage's
+// variant, or -1 when the cursor is at the end of the stream or blocked.  Only LDS is
+// read besides the stage itself.
+template <class AT, bool KN>
+__device__ __forceinline__ int load_at_cursor(Stream<AT>& st, const QueueLds& q,
+                                              u4_t (&dst)[StreamGeom<AT>::NP]) {
+    if (!st.ct.valid || st.blocked) return -1;
+    stream_load<AT, KN>(st.ct, st.cc, dst);
+    const int v = st.ct.variant;
+    ++st.loaded;
+    if (++st.cc >= st.ct.nch) {
+        st.cc = 0;
+        ++st.tc;
+        if (st.tc < st.fetched)
+            st.ct = stream_tile<AT>(q, st.tc);
+        else
+            st.blocked = true;
     }
     return v;
 }
 
-template <class AT, class V>
-__device__ __forceinline__ void run_tile(const BatchArgs& a, Stream<AT>& st, unsigned char* smem, bool b_kn) {
+// Slot S holds the stage after the one being computed: write it to LDS buffer S and
+// load the stage at the cursor into it.
+template <class AT, bool KN, int S>
+__device__ __forceinline__ void advance_slot(Stream<AT>& st, const QueueLds& q, unsigned char* smem) {
+    using SG = StreamGeom<AT>;
+    auto& slot = S ? st.slot1 : st.slot0;
+    int& sv = S ? st.sv1 : st.sv0;
+    if (sv >= 0) stream_write<AT, KN>(slot, smem + S * SG::STAGE);
+    sv = load_at_cursor<AT, KN>(st, q, slot);
+}
+
+// Loads stage `loaded` into its slot (runtime parity; tile-boundary use only).
+template <class AT, bool KN>
+__device__ __forceinline__ void load_next_into_slot(Stream<AT>& st, const QueueLds& q) {
+    if (st.loaded & 1)
+        st.sv1 = load_at_cursor<AT, KN>(st, q, st.slot1);
+    else
+        st.sv0 = load_at_cursor<AT, KN>(st, q, st.slot0);
+}
+
+template <class AT, bool KN, class V>
+__device__ __forceinline__ void run_tile(const BatchArgs& a, Stream<AT>& st, const QueueLds& q,
+                                         unsigned char* smem) {
     using SG = StreamGeom<AT>;
     using acc_t = typename AT::acc_t;
     using out_t = typename AT::out_t;
     using frag_t = typename AT::frag_t;
     using accv_t = typename AT::accv_t;
     constexpr int FB = AT::FRAG_BYTES;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = uni(threadIdx.x >> 6);
     const int wm = wave / V::WN, wn = wave % V::WN;
-    const TileRef cur = st.T[0];
-    const kami_problem P = a.probs[cur.prob];
-    const int mlim = P.m - cur.row0, nlim = P.n - cur.col0;
+    const Tile cur = st.cur;
+    // wave-uniform bounds
+    const int mlim = uni(cur.mlim), nlim = uni(cur.nlim), nch = uni(cur.nch);
 
     accv_t acc[V::IM][V::JN];
 #pragma unroll
@@ -244,51 +387,74 @@ __device__ __forceinline__ void run_tile(const BatchArgs& a, Stream<AT>& st, uns
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[i][j][v] = 0;
 
-    for (int c = 0; c < cur.nch; ++c) {
-        const unsigned char* base = smem + st.buf * SG::STAGE;
+    // Every atom of a live wave is computed: staged rows and columns outside C are zero,
+    // and a guard per atom would make the compiler copy the accumulators around each
+    // MFMA.  Waves whose whole sub-tile lies outside C skip the products.
+    const bool wave_live = wm * V::SUBM < mlim && wn * V::SUBN < nlim;  // wave-uniform
+    auto compute = [&](const unsigned char* base) {
         const unsigned char* bb = base + SG::A_REGION;
+        if (!wave_live) return;
 #pragma unroll
         for (int ks = 0; ks < SG::KC / AT::KSTEP; ++ks) {
             const int kk = (ks * AT::KSTEP + AT::KV * (lane >> 4)) * SG::ELT;
-            frag_t af[V::IM];
+            frag_t af[V::IM], bf[V::JN];
 #pragma unroll
             for (int im = 0; im < V::IM; ++im)
-                af[im] = bitcast<frag_t>(
-                    load_raw<FB>(base + (wm * V::SUBM + im * 16 + (lane & 15)) * SG::ROWB + kk));
+                af[im] = bitcast<frag_t>(load_raw<FB>(
+                    base + (wm * V::SUBM + im * 16 + (lane & 15)) * SG::ROWB + kk));
 #pragma unroll
-            for (int jn = 0; jn < V::JN; ++jn) {
-                if (wn * V::SUBN + jn * 16 >= nlim) continue;  // atom outside C: skip
-                const frag_t bf = bitcast<frag_t>(
-                    load_raw<FB>(bb + (wn * V::SUBN + jn * 16 + (lane & 15)) * SG::ROWB + kk));
+            for (int jn = 0; jn < V::JN; ++jn)
+                bf[jn] = bitcast<frag_t>(load_raw<FB>(
+                    bb + (wn * V::SUBN + jn * 16 + (lane & 15)) * SG::ROWB + kk));
+            // sub-steps of a fragment (FP64: two 16x16x4 instructions) interleaved across
+            // the accumulators, so no instruction waits on the one just issued
 #pragma unroll
-                for (int im = 0; im < V::IM; ++im)
-                    if (wm * V::SUBM + im * 16 < mlim) AT::mma(acc[im][jn], af[im], bf);
-            }
+            for (int e = 0; e < AT::NSUB; ++e)
+#pragma unroll
+                for (int jn = 0; jn < V::JN; ++jn)
+#pragma unroll
+                    for (int im = 0; im < V::IM; ++im)
+                        AT::mma_sub(acc[im][jn], af[im], bf[jn], e);
         }
-        // the stage after this one goes to the other buffer; the ring shifts and the
-        // stage two ahead (possibly of a later tile) is loaded
-        if (st.v0 >= 0) stream_write<AT>(st.v0, b_kn, st.ring0, smem + (st.buf ^ 1) * SG::STAGE);
-#pragma unroll
-        for (int i = 0; i < SG::NP; ++i) st.ring0[i] = st.ring1[i];
-        st.v0 = st.v1;
-        st.v1 = load_at_cursor<AT>(a, st, b_kn, st.ring1);
+    };
+    // stage of parity P: products from LDS buffer P, then the next stage (slot 1 - P) to
+    // LDS buffer 1 - P and the cursor's stage into that slot
+    auto stage0 = [&]() {
+        compute(smem);
+        advance_slot<AT, KN, 1>(st, q, smem);
         __syncthreads();
-        st.buf ^= 1;
+        ++st.x;
+    };
+    auto stage1 = [&]() {
+        compute(smem + SG::STAGE);
+        advance_slot<AT, KN, 0>(st, q, smem);
+        __syncthreads();
+        ++st.x;
+    };
+    int c = 0;
+    if ((st.x & 1) && c < nch) {
+        stage1();
+        ++c;
     }
+    for (; c + 1 < nch; c += 2) {
+        stage0();
+        stage1();
+    }
+    if (c < nch) stage0();
 
     const acc_t alpha = (acc_t)a.alpha, beta = (acc_t)a.beta;
-    out_t* C = reinterpret_cast<out_t*>(P.C);
+    out_t* C = reinterpret_cast<out_t*>(cur.C);
 #pragma unroll
     for (int im = 0; im < V::IM; ++im)
 #pragma unroll
         for (int jn = 0; jn < V::JN; ++jn) {
             const int rb = wm * V::SUBM + im * 16, cb = wn * V::SUBN + jn * 16;
             if (rb >= mlim || cb >= nlim) continue;
-            const int col = cur.col0 + cb + (lane & 15);
+            const int col = cb + (lane & 15);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = cur.row0 + rb + AT::out_row(lane, v);
-                out_t* dst = C + (int64_t)row * P.ldc + col;
+                const int row = rb + AT::out_row(lane, v);
+                out_t* dst = C + (int64_t)row * cur.ldc + col;
                 acc_t val = alpha * acc[im][jn][v];
                 if (beta != (acc_t)0) val += beta * (acc_t)(*dst);
                 *dst = (out_t)val;
@@ -296,48 +462,77 @@ __device__ __forceinline__ void run_tile(const BatchArgs& a, Stream<AT>& st, uns
         }
 }
 
-template <class AT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) batched_persistent_kernel(BatchArgs args) {
+template <class AT, bool KN>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+batched_persistent_kernel(BatchArgs args) {
     using SG = StreamGeom<AT>;
     using VS = Variants<AT>;
     __shared__ __attribute__((aligned(16))) unsigned char smem[SG::LDS];
-    __shared__ unsigned int s_item;
-    const bool b_kn = args.b_kn != 0;
+    __shared__ QueueLds q;
     Stream<AT> st;
-    st.T[0] = fetch_tile(args, &s_item);
-    if (!st.T[0].valid) return;  // uniform
-    st.T[1] = fetch_tile(args, &s_item);
-    st.T[2] = fetch_tile(args, &s_item);
-    set_chunks<AT>(args, st.T[0]);
-    set_chunks<AT>(args, st.T[1]);
-    set_chunks<AT>(args, st.T[2]);
-    st.lt = 0;
-    st.lc = 0;
-    st.buf = 0;
-    // prologue: first stage into LDS buffer 0, the next two into the ring
-    const int vfirst = load_at_cursor<AT>(args, st, b_kn, st.ring0);
-    stream_write<AT>(vfirst, b_kn, st.ring0, smem);
-    st.v0 = load_at_cursor<AT>(args, st, b_kn, st.ring0);
-    st.v1 = load_at_cursor<AT>(args, st, b_kn, st.ring1);
+    st.t0 = st.tc = 0;
+    st.cc = 0;
+    st.blocked = false;
+    st.x = st.loaded = 0;
+    st.fetched = 0;
+    st.phase = -1;
+    st.fvalid = false;
+    st.sv0 = st.sv1 = -1;
+    // prologue: the first batch synchronously, the second one started
+    ensure_ready(args, st, q, 0);
+    st.cur = stream_tile<AT>(q, 0);
+    if (!st.cur.valid) return;  // uniform: the queue is empty
+    st.ct = st.cur;
+    st.phase = 0;
+    fetch_step(args, st, q);  // atomic for the next batch
+    // stage 0 into LDS buffer 0, stage 1 into slot 1, stage 2 into slot 0
+    load_at_cursor<AT, KN>(st, q, st.slot0);
+    stream_write<AT, KN>(st.slot0, smem);
+    st.sv1 = load_at_cursor<AT, KN>(st, q, st.slot1);
+    st.sv0 = load_at_cursor<AT, KN>(st, q, st.slot0);
     __syncthreads();
-    while (st.T[0].valid) {
-        switch (st.T[0].variant) {
-            case 0: run_tile<AT, typename VS::V0>(args, st, smem, b_kn); break;
-            case 1: run_tile<AT, typename VS::V1>(args, st, smem, b_kn); break;
-            case 2: run_tile<AT, typename VS::V2>(args, st, smem, b_kn); break;
-            default: run_tile<AT, typename VS::V3>(args, st, smem, b_kn); break;
+    while (true) {
+        switch (st.cur.variant) {
+            case 0: run_tile<AT, KN, typename VS::V0>(args, st, q, smem); break;
+            case 1: run_tile<AT, KN, typename VS::V1>(args, st, q, smem); break;
+            case 2: run_tile<AT, KN, typename VS::V2>(args, st, q, smem); break;
+            default:
+                if constexpr (SG::TMAX >= 128) run_tile<AT, KN, typename VS::V3>(args, st, q, smem);
+                break;
         }
-        // the next queued tile becomes current; one more is popped behind it
-        st.T[0] = st.T[1];
-        st.T[1] = st.T[2];
-        st.T[2] = fetch_tile(args, &s_item);
-        set_chunks<AT>(args, st.T[2]);
-        st.lt -= 1;
-        // the cursor may have stopped at the end of the queue; if a tile appeared, the
-        // ring is refilled from it
-        if (st.v1 < 0 && st.lt >= 0) {
-            if (st.v0 < 0) st.v0 = load_at_cursor<AT>(args, st, b_kn, st.ring0);
-            if (st.v0 >= 0) st.v1 = load_at_cursor<AT>(args, st, b_kn, st.ring1);
+        // ---- tile boundary ----
+        ++st.t0;
+        // one step of the batch fetch in flight; a new fetch starts when the current
+        // tile enters a new batch (the buffer it fills is the one two batches back)
+        if (st.phase < 0 && st.t0 % kQueueBatch == 0 && st.fetched == st.t0 + kQueueBatch)
+            st.phase = 0;
+        if (st.phase >= 0) {
+            fetch_step(args, st, q);
+            __syncthreads();
+            fetch_commit(st);
+        }
+        ensure_ready(args, st, q, st.t0);  // uniform; normally already done
+        st.cur = stream_tile<AT>(q, st.t0);
+        if (!st.cur.valid) break;  // uniform
+        // a blocked cursor resumes; stages that could not be loaded are loaded now
+        if (st.blocked) {
+            ensure_ready(args, st, q, st.tc);
+            st.ct = stream_tile<AT>(q, st.tc);
+            st.blocked = false;
+            if (st.loaded == st.x) {
+                // the new tile's first stage: through its slot straight to its LDS buffer
+                load_next_into_slot<AT, KN>(st, q);
+                if (st.x & 1) {
+                    stream_write<AT, KN>(st.slot1, smem + SG::STAGE);
+                    st.sv1 = -1;
+                } else {
+                    stream_write<AT, KN>(st.slot0, smem);
+                    st.sv0 = -1;
+                }
+            }
+            // then stages x + 1 and x + 2 into their slots
+            while (st.loaded < st.x + 3 && st.ct.valid) load_next_into_slot<AT, KN>(st, q);
+            __syncthreads();
         }
     }
 }
@@ -354,10 +549,10 @@ static int bucket_of(const kami_problem& p) {
 // Prior variant of a bucket: the largest tile that the problem fills, following the
 // reference's choice of fewer, larger warps for larger sizes
 // (scripts/fp64_batched_GH200.sh:7-14).
-static int prior_variant(int bucket) {
+static int prior_variant(int prec, int bucket) {
     const int cm = bucket / 25, cn = (bucket / 5) % 5;
     if (cm == 0 && cn == 0) return 1;
-    if (cm >= 3 && cn >= 2) return 3;
+    if (cm >= 3 && cn >= 2 && variants_of(prec) > 3) return 3;
     return 0;
 }
 
@@ -373,7 +568,6 @@ struct kami_batched_plan {
     double flops = 0;
     int b_kn = 0;
     double alpha = 1.0, beta = 0.0;
-    kami_problem* d_probs = nullptr;
     WorkItem* d_items = nullptr;
     unsigned int* d_counter = nullptr;
     bool use_graph = false;
@@ -386,50 +580,76 @@ struct kami_batched_plan {
 
 namespace kami {
 
-static hipError_t launch_batched(int prec, const BatchArgs& a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, a.counter);
-    if (prec == KAMI_PREC_F64)
-        hipLaunchKernelGGL(batched_persistent_kernel<AtomF64>, dim3(grid), dim3(256), 0, s, a);
-    else if (prec == KAMI_PREC_F16_F32)
-        hipLaunchKernelGGL(batched_persistent_kernel<AtomF16x32>, dim3(grid), dim3(256), 0, s, a);
+template <class AT>
+static hipError_t launch_for(const BatchArgs& a, int grid, hipStream_t s) {
+    if (a.b_kn)
+        hipLaunchKernelGGL((batched_persistent_kernel<AT, true>), dim3(grid), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL(batched_persistent_kernel<AtomBF16x32>, dim3(grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((batched_persistent_kernel<AT, false>), dim3(grid), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
-static hipError_t persistent_grid(int prec, int* grid) {
+static hipError_t launch_batched(int prec, const BatchArgs& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, a.counter);
+    if (prec == KAMI_PREC_F64) return launch_for<AtomF64>(a, grid, s);
+    if (prec == KAMI_PREC_F16_F32) return launch_for<AtomF16x32>(a, grid, s);
+    return launch_for<AtomBF16x32>(a, grid, s);
+}
+
+template <class AT>
+static hipError_t occupancy_for(bool kn, int* per_cu) {
+    if (kn)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, batched_persistent_kernel<AT, true>, 256, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, batched_persistent_kernel<AT, false>, 256, 0);
+}
+
+static hipError_t persistent_grid(int prec, bool kn, int* grid) {
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
     if (prec == KAMI_PREC_F64)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batched_persistent_kernel<AtomF64>, 256, 0);
+        e = occupancy_for<AtomF64>(kn, &per_cu);
     else if (prec == KAMI_PREC_F16_F32)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batched_persistent_kernel<AtomF16x32>, 256, 0);
+        e = occupancy_for<AtomF16x32>(kn, &per_cu);
     else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batched_persistent_kernel<AtomBF16x32>, 256, 0);
+        e = occupancy_for<AtomBF16x32>(kn, &per_cu);
     if (e != hipSuccess) return e;
     *grid = std::max(1, cus * std::max(1, per_cu));
     return hipSuccess;
 }
 
-// Cut every problem into tiles of its bucket's variant; longest tiles first.
+// Queue entries of the problems: every problem cut into tiles of its bucket's variant,
+// each entry carrying the tile's own operand pointers; longest first (by m n k of the
+// tile) so the tail of the queue holds the cheapest tiles.
 static void build_items(const std::vector<kami_problem>& probs, const std::vector<int>& bucket,
-                        const int* variant_of_bucket, std::vector<WorkItem>& items) {
+                        const int* variant_of_bucket, int ei, int eo, bool b_kn,
+                        std::vector<WorkItem>& items) {
     struct Keyed {
         int64_t cost;
         WorkItem w;
     };
     std::vector<Keyed> tmp;
     for (size_t p = 0; p < probs.size(); ++p) {
+        const kami_problem& P = probs[p];
         const int v = variant_of_bucket[bucket[p]];
         const int tm = kVariantTM[v], tn = kVariantTN[v];
-        for (int r = 0; r < probs[p].m; r += tm)
-            for (int c = 0; c < probs[p].n; c += tn) {
-                const int64_t cost = (int64_t)std::min(tm, probs[p].m - r) *
-                                     std::min(tn, probs[p].n - c) * probs[p].k;
-                tmp.push_back({cost, WorkItem{(int32_t)p, r, c, v}});
+        for (int r = 0; r < P.m; r += tm)
+            for (int c = 0; c < P.n; c += tn) {
+                WorkItem w;
+                w.A = static_cast<const char*>(P.A) + (int64_t)r * P.lda * ei;
+                w.B = static_cast<const char*>(P.B) + (b_kn ? (int64_t)c : (int64_t)c * P.ldb) * ei;
+                w.C = static_cast<char*>(P.C) + ((int64_t)r * P.ldc + c) * eo;
+                w.lda = P.lda;
+                w.ldb = P.ldb;
+                w.ldc = P.ldc;
+                w.mlim = P.m - r;
+                w.nlim = P.n - c;
+                w.k = P.k;
+                w.variant = v;
+                const int64_t cost = (int64_t)std::min(tm, P.m - r) * std::min(tn, P.n - c) * P.k;
+                tmp.push_back({cost, w});
             }
     }
     std::stable_sort(tmp.begin(), tmp.end(),
@@ -457,7 +677,6 @@ static void free_plan(kami_batched_plan* p) {
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
-    if (p->d_probs) (void)hipFree(p->d_probs);
     if (p->d_items) (void)hipFree(p->d_items);
     if (p->d_counter) (void)hipFree(p->d_counter);
     delete p;
@@ -519,23 +738,28 @@ kami_status kami_batched_plan_create(const kami_problem* probs, int64_t count, i
     }
     for (int b = 0; b < 125; ++b) {
         const int tuned = batched_tuned_variant(precision, b);
-        variant_of_bucket[b] = tuned >= 0 ? tuned : prior_variant(b);
+        variant_of_bucket[b] = tuned >= 0 && tuned < variants_of(precision) ? tuned
+                                                                            : prior_variant(precision, b);
     }
 
-    hipError_t e = persistent_grid(precision, &plan->grid);
+    hipError_t e = persistent_grid(precision, plan->b_kn, &plan->grid);
     if (e != hipSuccess) {
         free_plan(plan);
         return hip_fail(e);
     }
-    st = upload(hp.data(), hp.size() * sizeof(kami_problem), (void**)&plan->d_probs);
-    if (st == KAMI_OK) st = upload(nullptr, 0, (void**)&plan->d_counter);
+    st = upload(nullptr, 0, (void**)&plan->d_counter);
     if (st != KAMI_OK) {
         free_plan(plan);
         return st;
     }
 
     if (o.autotune) {
-        // time every variant on up to 256 problems of each bucket, keep the fastest
+        // time every variant on up to 256 problems of each bucket, keep the fastest.  The
+        // sample is repeated until it carries at least kTuneProblems problems so that every
+        // variant is timed at full occupancy (a small sample would be latency-bound and
+        // favour the smallest tiles); the repeats write identical values to the same C
+        // (tuning requires beta = 0).
+        constexpr size_t kTuneProblems = 4096;
         hipEvent_t e0, e1;
         e = hipEventCreate(&e0);
         if (e == hipSuccess) e = hipEventCreate(&e1);
@@ -552,19 +776,20 @@ kami_status kami_batched_plan_create(const kami_problem* probs, int64_t count, i
                     sample.push_back(hp[i]);
                     sb.push_back(0);
                 }
-            kami_problem* d_sample = nullptr;
-            st = upload(sample.data(), sample.size() * sizeof(kami_problem), (void**)&d_sample);
-            if (st != KAMI_OK) break;
+            const size_t distinct = sample.size();
+            for (size_t i = 0; sample.size() < kTuneProblems; ++i) {
+                sample.push_back(sample[i % distinct]);
+                sb.push_back(0);
+            }
             float best = 1e30f;
-            for (int v = 0; v < kVariants; ++v) {
+            for (int v = 0; v < variants_of(precision); ++v) {
                 int vb[1] = {v};
                 std::vector<WorkItem> items;
-                build_items(sample, sb, vb, items);
+                build_items(sample, sb, vb, ei, eo, plan->b_kn, items);
                 WorkItem* d_it = nullptr;
                 st = upload(items.data(), items.size() * sizeof(WorkItem), (void**)&d_it);
                 if (st != KAMI_OK) break;
-                BatchArgs a{d_sample, d_it, plan->d_counter, (int64_t)items.size(), o.alpha, 0.0,
-                            plan->b_kn};
+                BatchArgs a{d_it, plan->d_counter, (int64_t)items.size(), o.alpha, 0.0, plan->b_kn};
                 e = launch_batched(precision, a, plan->grid, s);  // warm-up
                 if (e == hipSuccess) e = hipEventRecord(e0, s);
                 for (int rep = 0; rep < 3 && e == hipSuccess; ++rep)
@@ -583,7 +808,6 @@ kami_status kami_batched_plan_create(const kami_problem* probs, int64_t count, i
                     variant_of_bucket[b] = v;
                 }
             }
-            (void)hipFree(d_sample);
             if (st == KAMI_OK) set_batched_tuned_variant(precision, b, variant_of_bucket[b]);
         }
         (void)hipEventDestroy(e0);
@@ -595,7 +819,7 @@ kami_status kami_batched_plan_create(const kami_problem* probs, int64_t count, i
     }
 
     std::vector<WorkItem> items;
-    build_items(hp, bucket, variant_of_bucket, items);
+    build_items(hp, bucket, variant_of_bucket, ei, eo, plan->b_kn, items);
     plan->n_items = (int64_t)items.size();
     st = upload(items.data(), items.size() * sizeof(WorkItem), (void**)&plan->d_items);
     if (st != KAMI_OK) {
@@ -612,7 +836,7 @@ kami_status kami_batched_plan_create(const kami_problem* probs, int64_t count, i
         e = hipStreamCreateWithFlags(&plan->cap_stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipStreamBeginCapture(plan->cap_stream, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            BatchArgs a{plan->d_probs, plan->d_items, plan->d_counter, plan->n_items, plan->alpha,
+            BatchArgs a{plan->d_items, plan->d_counter, plan->n_items, plan->alpha,
                         plan->beta, plan->b_kn};
             hipError_t le = launch_batched(precision, a, plan->grid, plan->cap_stream);
             e = hipStreamEndCapture(plan->cap_stream, &plan->graph);
@@ -635,7 +859,7 @@ kami_status kami_batched_execute(kami_batched_plan* plan, void* stream) {
     if (plan->use_graph) {
         e = hipGraphLaunch(plan->exec, s);
     } else {
-        BatchArgs a{plan->d_probs, plan->d_items, plan->d_counter, plan->n_items, plan->alpha,
+        BatchArgs a{plan->d_items, plan->d_counter, plan->n_items, plan->alpha,
                     plan->beta, plan->b_kn};
         e = launch_batched(plan->prec, a, plan->grid, s);
     }

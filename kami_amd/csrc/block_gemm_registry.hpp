@@ -78,10 +78,12 @@ constexpr int choose_ks_pipe(int budget) {
     return best;
 }
 
-// Persistent 2D instance (PM x PN waves) with prefetch depth up to PD_WANT.
-template <class AT, int PREC, int BM, int BN, int BK, int PM, int PN, int PD_WANT>
+// Persistent 2D instance (PM x PN waves) with prefetch depth up to PD_WANT and K stages
+// sized to an LDS budget of BUDGET bytes for the two buffers.
+template <class AT, int PREC, int BM, int BN, int BK, int PM, int PN, int PD_WANT,
+          int BUDGET = kStageBudget>
 void add_pipe_instance(std::vector<BlockInstance>& v, int ranks) {
-    constexpr int KS = choose_ks_pipe<AT, BM, BN, BK, PM, PN>(kStageBudget);
+    constexpr int KS = choose_ks_pipe<AT, BM, BN, BK, PM, PN>(BUDGET);
     using G = BlockGeom<AT, BM, BN, BK, PM, PN, 1, KS>;
     constexpr int PD = pick_depth(G::NS, PD_WANT);
     v.push_back(BlockInstance{PREC, BM, BN, BK, 2, ranks, PM, PN, 1, KS, G::NT, 2 * G::STAGE_LDS,

@@ -168,17 +168,20 @@ kami_status validate_desc(const kami_gemm_desc* d) {
     return KAMI_OK;
 }
 
-// Streaming-mode winners of the instance sweep on MI355X (one problem per workgroup,
-// 4096 FP64 / 8192 FP16 problems); used for KAMI_PART_AUTO until
-// kami_block_gemm_autotune has measured the shape in this process.
+// Streaming-mode winners of the instance sweep on MI355X (4096 FP64 / 8192 FP16 problems;
+// for FP64 the persistent 2D form, pipe = 2, wins at 32^3..128^3: 15.5 / 27.2 / 52.1
+// TFLOPS); used for KAMI_PART_AUTO until kami_block_gemm_autotune has measured the shape
+// in this process (or a tuning file has been loaded).
 struct Prior {
     int prec, m, n, k, part, ranks, pipe = 1;
 };
 static const Prior kPriors[] = {
-    {KAMI_PREC_F64, 16, 16, 16, KAMI_PART_2D, 1},   {KAMI_PREC_F64, 32, 32, 32, KAMI_PART_3D, 2},
-    {KAMI_PREC_F64, 64, 64, 64, KAMI_PART_1D, 4},   {KAMI_PREC_F64, 128, 128, 128, KAMI_PART_3D, 2},
-    {KAMI_PREC_F16_F32, 32, 32, 32, KAMI_PART_2D, 1}, {KAMI_PREC_F16_F32, 64, 64, 64, KAMI_PART_2D, 4},
-    {KAMI_PREC_F16_F32, 128, 128, 128, KAMI_PART_1D, 4},
+    {KAMI_PREC_F64, 16, 16, 16, KAMI_PART_2D, 1},
+    {KAMI_PREC_F64, 32, 32, 32, KAMI_PART_2D, 2, 2},
+    {KAMI_PREC_F64, 64, 64, 64, KAMI_PART_2D, 4, 2},
+    {KAMI_PREC_F64, 128, 128, 128, KAMI_PART_2D, 4, 2},
+    {KAMI_PREC_F16_F32, 32, 32, 32, KAMI_PART_2D, 1}, {KAMI_PREC_F16_F32, 64, 64, 64, KAMI_PART_1D, 4},
+    {KAMI_PREC_F16_F32, 128, 128, 128, KAMI_PART_2D, 4},
     {KAMI_PREC_F16_F32, 256, 256, 256, KAMI_PART_1D, 16},
     {KAMI_PREC_F16_F32, 256, 256, 16, KAMI_PART_1D, 16},
     {KAMI_PREC_F16_F32, 256, 256, 32, KAMI_PART_1D, 16},

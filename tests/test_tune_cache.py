@@ -27,7 +27,7 @@ def save(path):
 
 def test_loaded_block_choice_drives_auto(tmp_path):
     before = K.query(64, 64, 64, K.PREC_F64)
-    assert (before["partition"], before["ranks"]) == (1, 4)   # the measured prior: 1D, P = 4
+    assert before["partition"] != 3                           # the measured prior is not 3D
     f = tmp_path / "t.txt"
     f.write_text("kami-tune 1\nblock 0 64 64 64 3 2 1\n")
     assert load(f) == _abi.OK
