@@ -76,6 +76,23 @@ def cpu_baseline_f64(m, n, k, count, budget_s=10.0):
                       f"triple loop, OpenMP over problems, {cores} threads), {dt:.1f} s"}
 
 
+def measured_traffic(path, cfg, m, count):
+    """HBM bytes per launch of the path's kernel from the PMC record in profiles/traffic.json
+    (FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy), when that record was taken
+    on the same instance and workload; otherwise None."""
+    rec = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            rec = json.load(f).get({"native": "w3_native", "split": "w3_split"}.get(path, ""))
+    except (OSError, ValueError):
+        return None
+    if not rec or m != 128 or count != 4096:
+        return None
+    if any(cfg.get(k) != v for k, v in rec.get("config", {}).items()):
+        return None
+    return rec["traffic_bytes"]
+
+
 def rel_errors(C, A, B, sample):
     """Max errors of C against the compensated (Dot2) oracle on the first `sample`
     problems: componentwise |dC| / (|A||B|) and normwise max|dC| / max|C*| per problem."""
@@ -90,8 +107,8 @@ def rel_errors(C, A, B, sample):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--path", choices=["auto", "native", "split", "split_fullk"], default="auto")
     ap.add_argument("--m", type=int, default=128)
     ap.add_argument("--problems", type=int, default=4096)
@@ -159,13 +176,15 @@ def main():
             sys.exit(1)
     r = results[chosen]
     ms = r["ms"]
+    traffic = measured_traffic(chosen, r["config"], m, count)
     if chosen == "native":
         roof = {"bound": "mfma", "achieved": r["tflops"], "peak": PEAK["f64_mfma_tflops"],
-                "unit": "TFLOP/s", "frac": r["tflops"] / PEAK["f64_mfma_tflops"], "traffic": None}
+                "unit": "TFLOP/s", "frac": r["tflops"] / PEAK["f64_mfma_tflops"],
+                "traffic": traffic}
     else:
         gbs = r["hbm_gbs"]
         roof = {"bound": "hbm", "achieved": gbs, "peak": PEAK["hbm_gbs"], "unit": "GB/s",
-                "frac": gbs / PEAK["hbm_gbs"], "traffic": None}
+                "frac": gbs / PEAK["hbm_gbs"], "traffic": traffic}
     cpu = None if args.no_cpu else cpu_baseline_f64(m, n, k, count)
     line = {
         "metric": METRIC,
