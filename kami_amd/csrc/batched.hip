@@ -713,6 +713,13 @@ kami_status kami_batched_plan_create(const kami_problem* probs, int64_t count, i
         if ((p.lda * ei) % 16 || (p.ldb * ei) % 16 || ((uintptr_t)p.A % 16) ||
             ((uintptr_t)p.B % 16) || ((uintptr_t)p.C % eo))
             return KAMI_EINVAL;
+        // Operand pieces are addressed by 32-bit byte offsets from the tile's origin
+        // (stream_load): the farthest piece of any tile must end below kBufferRange.
+        const int64_t a_span = ((int64_t)(std::min(p.m, 128) - 1) * p.lda + p.k) * ei;
+        const int64_t b_span = o.b_layout == KAMI_B_NK
+                                   ? ((int64_t)(std::min(p.n, 64) - 1) * p.ldb + p.k) * ei
+                                   : ((int64_t)(p.k - 1) * p.ldb + std::min(p.n, 64)) * ei;
+        if (a_span > kBufferRange || b_span > kBufferRange) return KAMI_ENOTSUP;
         flops += 2.0 * p.m * p.n * (double)p.k;
     }
     kami_status st = device_check();

@@ -135,6 +135,20 @@ def test_batched_plan_validation_without_device():
                                         ctypes.byref(h)) == _abi.ENOTSUP
     assert lib.kami_batched_plan_create(arr, 0, K.PREC_F64, ctypes.byref(opts), None,
                                         ctypes.byref(h)) == _abi.EINVAL
+    # A tile's operand pieces are addressed by 32-bit byte offsets: 32 rows of A 2^24
+    # doubles apart span 31 * 2^27 bytes > 2^31 - 16.
+    arr[0].lda = 1 << 24
+    assert lib.kami_batched_plan_create(arr, 2, K.PREC_F64, ctypes.byref(opts), None,
+                                        ctypes.byref(h)) == _abi.ENOTSUP
+    if lib.kami_device_check() == _abi.ENODEV:
+        arr[0].lda = 1 << 20  # 31 * 2^23 bytes fits: the request reaches the device check
+        assert lib.kami_batched_plan_create(arr, 2, K.PREC_F64, ctypes.byref(opts), None,
+                                            ctypes.byref(h)) == _abi.ENODEV
+    arr[0].lda = 0
+    opts.b_layout = K.B_KN
+    arr[1].ldb = 1 << 26  # K x N B: 31 rows of 2^26 halves = 31 * 2^27 bytes
+    assert lib.kami_batched_plan_create(arr, 2, K.PREC_F16_F32, ctypes.byref(opts), None,
+                                        ctypes.byref(h)) == _abi.ENOTSUP
 
 
 def test_spmm_validation_without_device():
