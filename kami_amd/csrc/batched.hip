@@ -6,10 +6,11 @@
 // strides (:71-74) with the partition picked per size by the sweep script
 // (scripts/fp64_batched_GH200.sh:96-123).  Here:
 //   * problems carry their own {m, n, k, lda, ldb, ldc, A, B, C} (the kami_problem array);
-//   * every problem is cut into output tiles; a tile is one queue entry
-//     {problem, row0, col0, variant};
-//   * a persistent grid (CUs x resident workgroups) pops entries with one atomic add
-//     per tile, in descending-cost order (longest first), until the queue is empty;
+//   * every problem is cut into output tiles; a tile is one 64-byte queue entry holding
+//     its own operand pointers, strides, extents and variant (WorkItem);
+//   * a persistent grid (CUs x resident workgroups) pops kQueueBatch entries per atomic
+//     add, in descending-cost order (longest first), until the queue is empty; the next
+//     batch is fetched while the current one's tiles compute;
 //   * the variant of a tile is the tile shape and wave partition of its size bucket,
 //     picked by timing candidates on the device at plan creation (or from priors);
 //   * one execution is [reset the queue counter] -> [persistent kernel], captured once
@@ -47,9 +48,9 @@ template <class AT>
 struct StreamGeom {
     static constexpr int NT = 256;
     static constexpr int ELT = (int)sizeof(typename AT::in_t), VE = 16 / ELT;
-    // FP64 stages 32-wide K chunks of tiles up to 64 x 64 (the 128-row variant is not
-    // offered), FP16/BF16 64-wide chunks of tiles up to 128 x 64: about 64 KB / 55 KB of
-    // LDS for the two stage buffers, two workgroups per CU
+    // FP64 stages 16-wide K chunks of tiles up to 64 x 64 (the 128-row variant is not
+    // offered), FP16/BF16 64-wide chunks of tiles up to 128 x 64: 36 KB / 54 KB of LDS for
+    // the two stage buffers, two workgroups per CU
     static constexpr int KC = ELT == 8 ? 16 : 64;
     static constexpr int TMAX = ELT == 8 ? 64 : 128, TNMAX = 64;
     static constexpr int ROWB = KC * ELT + 16;
