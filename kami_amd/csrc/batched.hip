@@ -8,9 +8,10 @@
 //   * problems carry their own {m, n, k, lda, ldb, ldc, A, B, C} (the kami_problem array);
 //   * every problem is cut into output tiles; a tile is one 64-byte queue entry holding
 //     its own operand pointers, strides, extents and variant (WorkItem);
-//   * a persistent grid (CUs x resident workgroups) pops kQueueBatch entries per atomic
-//     add, in descending-cost order (longest first), until the queue is empty; the next
-//     batch is fetched while the current one's tiles compute;
+//   * a persistent grid (CUs x resident workgroups) pops a batch of up to kQueueBatch
+//     entries per atomic add (fewer when the queue is short, so that every workgroup
+//     gets work), in descending-cost order (longest first), until the queue is empty;
+//     the next batch is fetched while the current one's tiles compute;
 //   * the variant of a tile is the tile shape and wave partition of its size bucket,
 //     picked by timing candidates on the device at plan creation (or from priors);
 //   * one execution is [reset the queue counter] -> [persistent kernel], captured once
@@ -38,12 +39,18 @@ struct WorkItem {
 };
 static_assert(sizeof(WorkItem) == 64, "queue entry layout");
 
-// Queue entries popped per atomic add.
-constexpr int kQueueBatch = 8;
+// Queue entries popped per atomic add: a power of two from 2 to kQueueBatch (the LDS
+// record size); the plan picks fewer for short queues.  Fetching a batch takes three
+// tile boundaries (atomic add, entry loads, LDS write), so the next batch is requested
+// when the workgroup starts the tile kFetchAhead before the end of its current batch,
+// and never at the batch's first tile: a workgroup that starts early cannot take a
+// second batch before the others have taken their first.
+constexpr int kQueueShift = 3, kQueueBatch = 1 << kQueueShift;
+constexpr int kFetchAhead = 3;
 
 // Staging shared by every tile variant: a stage holds up to TMAX rows of A and TNMAX rows
 // of B (or KC rows of TNMAX columns when B is K x N), KC elements of K each, in LDS rows
-// padded by 16 bytes.  Pieces are 16 bytes; a thread owns pieces tid + i * NT.
+// padded to the conflict-free stride.  Pieces are 16 bytes; a thread owns pieces tid + i * NT.
 template <class AT>
 struct StreamGeom {
     static constexpr int NT = 256;
@@ -53,7 +60,7 @@ struct StreamGeom {
     // the two stage buffers, two workgroups per CU
     static constexpr int KC = ELT == 8 ? 16 : 64;
     static constexpr int TMAX = ELT == 8 ? 64 : 128, TNMAX = 64;
-    static constexpr int ROWB = KC * ELT + 16;
+    static constexpr int ROWB = lds_row_bytes(KC * ELT, AT::FRAG_BYTES);
     static constexpr int A_REGION = TMAX * ROWB;
     static constexpr int STAGE = (TMAX + TNMAX) * ROWB;
     static constexpr int LDS = 2 * STAGE;
@@ -101,6 +108,7 @@ struct BatchArgs {
     int64_t n_items;
     double alpha, beta;
     int32_t b_kn;
+    int32_t qshift;  // entries per pop = 1 << qshift, at most kQueueBatch
 };
 
 // A tile as the kernel holds it: wave-uniform values (in scalar registers).
@@ -227,7 +235,7 @@ __device__ __forceinline__ void stream_write(const u4_t (&r)[StreamGeom<AT>::NP]
 }
 
 // Shared state of a workgroup's stream of (tile, K chunk) stages.
-//   * The queue is popped kQueueBatch entries at a time into one of two LDS record
+//   * The queue is popped Q = qb = 1 << qs entries at a time into one of two LDS record
 //     buffers; stream tile s lives in buffer (s / Q) % 2, slot s % Q.  The next batch
 //     is fetched in steps taken at tile boundaries (atomic add; entry loads; LDS write),
 //     so its latency hides behind whole tiles.
@@ -250,7 +258,9 @@ struct Stream {
     int64_t x;           // stage being computed
     int64_t loaded;      // stages loaded so far
     int64_t fetched;     // stream tiles whose records are in LDS (whole batches)
-    int phase;           // next batch fetch: 0 idle, 1 atomic issued, 2 entries loaded
+    int qb, qs, lead;    // entries per batch (1 << qs); tile of a batch that requests the next
+    int phase;           // next batch fetch: -1 idle, 0 start, 1 atomic issued,
+                         // 2 entries loaded, 3 records written
     u4_t slot0[StreamGeom<AT>::NP], slot1[StreamGeom<AT>::NP];
     int sv0, sv1;        // variant of the stage in each slot, -1 = empty
     // wave 0 only: batch fetch in flight
@@ -265,8 +275,8 @@ struct QueueLds {
 };
 
 template <class AT>
-__device__ __forceinline__ Tile stream_tile(const QueueLds& q, int64_t s) {
-    const int b = (int)((s / kQueueBatch) & 1), slot = (int)(s % kQueueBatch);
+__device__ __forceinline__ Tile stream_tile(const QueueLds& q, int64_t s, int qs) {
+    const int b = (int)((s >> qs) & 1), slot = (int)(s & ((1 << qs) - 1));
     return read_tile<AT>(&q.rec[b][slot], uni(q.valid[b][slot]) != 0);
 }
 
@@ -277,21 +287,21 @@ __device__ __forceinline__ void fetch_step(const BatchArgs& a, Stream<AT>& st, Q
     st.phase = uni(st.phase);
     if (st.phase == 0) {
         // pop the queue range of stream tiles [fetched, fetched + Q)
-        if (threadIdx.x == 0) st.fbase = atomicAdd(a.counter, (unsigned)kQueueBatch);
+        if (threadIdx.x == 0) st.fbase = atomicAdd(a.counter, (unsigned)st.qb);
         st.phase = 1;
     } else if (st.phase == 1) {
         if (threadIdx.x < 64) {
             const unsigned base = __shfl(st.fbase, 0);
             const int lane = threadIdx.x, e = lane >> 2;
-            st.fvalid = e < kQueueBatch && (int64_t)base + e < a.n_items;
+            st.fvalid = e < st.qb && (int64_t)base + e < a.n_items;
             if (st.fvalid)
                 st.fpiece = load_raw<16>(reinterpret_cast<const unsigned char*>(a.items + base + e) +
                                          16 * (lane & 3));
         }
         st.phase = 2;
     } else {
-        const int b = (int)((st.fetched / kQueueBatch) & 1);
-        if (threadIdx.x < 4 * kQueueBatch) {
+        const int b = (int)((st.fetched >> st.qs) & 1);
+        if (threadIdx.x < 4 * st.qb) {
             const int e = threadIdx.x >> 2;
             if (st.fvalid)
                 store_raw<16>(reinterpret_cast<unsigned char*>(&q.rec[b][e]) + 16 * (threadIdx.x & 3),
@@ -305,7 +315,7 @@ __device__ __forceinline__ void fetch_step(const BatchArgs& a, Stream<AT>& st, Q
 template <class AT>
 __device__ __forceinline__ void fetch_commit(Stream<AT>& st) {
     if (st.phase == 3) {
-        st.fetched += kQueueBatch;
+        st.fetched += st.qb;
         st.phase = -1;  // idle until the next batch is started
     }
 }
@@ -337,7 +347,7 @@ __device__ __forceinline__ int load_at_cursor(Stream<AT>& st, const QueueLds& q,
         st.cc = 0;
         ++st.tc;
         if (st.tc < st.fetched)
-            st.ct = stream_tile<AT>(q, st.tc);
+            st.ct = stream_tile<AT>(q, st.tc, st.qs);
         else
             st.blocked = true;
     }
@@ -463,8 +473,11 @@ __device__ __forceinline__ void run_tile(const BatchArgs& a, Stream<AT>& st, con
         }
 }
 
+// FP64 fits 168 VGPRs: three workgroups per CU (three fit in LDS); FP16/BF16 need more
+// registers, and two fit in LDS.
 template <class AT, bool KN>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(sizeof(typename AT::in_t) == 8 ? 3 : 2)))
 batched_persistent_kernel(BatchArgs args) {
     using SG = StreamGeom<AT>;
     using VS = Variants<AT>;
@@ -479,13 +492,14 @@ batched_persistent_kernel(BatchArgs args) {
     st.phase = -1;
     st.fvalid = false;
     st.sv0 = st.sv1 = -1;
-    // prologue: the first batch synchronously, the second one started
+    st.qs = min(kQueueShift, max(1, uni(args.qshift)));
+    st.qb = 1 << st.qs;
+    st.lead = max(1, st.qb - kFetchAhead);
+    // prologue: the first batch synchronously
     ensure_ready(args, st, q, 0);
-    st.cur = stream_tile<AT>(q, 0);
+    st.cur = stream_tile<AT>(q, 0, st.qs);
     if (!st.cur.valid) return;  // uniform: the queue is empty
     st.ct = st.cur;
-    st.phase = 0;
-    fetch_step(args, st, q);  // atomic for the next batch
     // stage 0 into LDS buffer 0, stage 1 into slot 1, stage 2 into slot 0
     load_at_cursor<AT, KN>(st, q, st.slot0);
     stream_write<AT, KN>(st.slot0, smem);
@@ -503,9 +517,12 @@ batched_persistent_kernel(BatchArgs args) {
         }
         // ---- tile boundary ----
         ++st.t0;
-        // one step of the batch fetch in flight; a new fetch starts when the current
-        // tile enters a new batch (the buffer it fills is the one two batches back)
-        if (st.phase < 0 && st.t0 % kQueueBatch == 0 && st.fetched == st.t0 + kQueueBatch)
+        // the next batch is requested at tile `lead` (at least 1) of the current one,
+        // which is then fetched; a fetch in flight takes one step per tile boundary (the
+        // buffer it fills is the one of the batch before the current one, which the
+        // tile and the cursor have left)
+        if (st.phase < 0 && (int)(st.t0 & (st.qb - 1)) == st.lead &&
+            st.fetched == st.t0 - st.lead + st.qb)
             st.phase = 0;
         if (st.phase >= 0) {
             fetch_step(args, st, q);
@@ -513,12 +530,12 @@ batched_persistent_kernel(BatchArgs args) {
             fetch_commit(st);
         }
         ensure_ready(args, st, q, st.t0);  // uniform; normally already done
-        st.cur = stream_tile<AT>(q, st.t0);
+        st.cur = stream_tile<AT>(q, st.t0, st.qs);
         if (!st.cur.valid) break;  // uniform
         // a blocked cursor resumes; stages that could not be loaded are loaded now
         if (st.blocked) {
             ensure_ready(args, st, q, st.tc);
-            st.ct = stream_tile<AT>(q, st.tc);
+            st.ct = stream_tile<AT>(q, st.tc, st.qs);
             st.blocked = false;
             if (st.loaded == st.x) {
                 // the new tile's first stage: through its slot straight to its LDS buffer
@@ -590,6 +607,15 @@ static hipError_t launch_for(const BatchArgs& a, int grid, hipStream_t s) {
     return hipGetLastError();
 }
 
+// log2 of the entries per pop: kQueueBatch when every workgroup gets three batches or
+// more, fewer for a short queue (4096 tiles over 768 workgroups: two entries per pop).
+static int32_t queue_shift(int64_t n_items, int grid) {
+    const int64_t per = n_items / (3 * (int64_t)std::max(grid, 1));
+    int32_t sh = 1;
+    while (sh < kQueueShift && (int64_t(2) << sh) <= per) ++sh;
+    return sh;
+}
+
 static hipError_t launch_batched(int prec, const BatchArgs& a, int grid, hipStream_t s) {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, a.counter);
     if (prec == KAMI_PREC_F64) return launch_for<AtomF64>(a, grid, s);
@@ -622,17 +648,23 @@ static hipError_t persistent_grid(int prec, bool kn, int* grid) {
 }
 
 // Queue entries of the problems: every problem cut into tiles of its bucket's variant,
-// each entry carrying the tile's own operand pointers; longest first (by m n k of the
-// tile) so the tail of the queue holds the cheapest tiles.
+// each entry carrying the tile's own operand pointers.  Problems are ordered longest
+// first (by m n k) so the tail of the queue holds the cheapest work, and a problem's tiles
+// stay consecutive in row-major order: a workgroup pops a batch of neighbouring tiles at
+// once, so tiles that share rows of A or columns of B run on one CU, back to back, and
+// find those operands in its L2 instead of fetching them again (ordering by tile cost
+// scattered them over all XCDs and read A and B 1.8 times over).
 static void build_items(const std::vector<kami_problem>& probs, const std::vector<int>& bucket,
                         const int* variant_of_bucket, int ei, int eo, bool b_kn,
                         std::vector<WorkItem>& items) {
-    struct Keyed {
-        int64_t cost;
-        WorkItem w;
-    };
-    std::vector<Keyed> tmp;
-    for (size_t p = 0; p < probs.size(); ++p) {
+    std::vector<int64_t> order(probs.size());
+    for (size_t p = 0; p < probs.size(); ++p) order[p] = (int64_t)p;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        const kami_problem &P = probs[a], &Q = probs[b];
+        return (int64_t)P.m * P.n * P.k > (int64_t)Q.m * Q.n * Q.k;
+    });
+    items.clear();
+    for (int64_t p : order) {
         const kami_problem& P = probs[p];
         const int v = variant_of_bucket[bucket[p]];
         const int tm = kVariantTM[v], tn = kVariantTN[v];
@@ -649,15 +681,9 @@ static void build_items(const std::vector<kami_problem>& probs, const std::vecto
                 w.nlim = P.n - c;
                 w.k = P.k;
                 w.variant = v;
-                const int64_t cost = (int64_t)std::min(tm, P.m - r) * std::min(tn, P.n - c) * P.k;
-                tmp.push_back({cost, w});
+                items.push_back(w);
             }
     }
-    std::stable_sort(tmp.begin(), tmp.end(),
-                     [](const Keyed& a, const Keyed& b) { return a.cost > b.cost; });
-    items.clear();
-    items.reserve(tmp.size());
-    for (auto& k : tmp) items.push_back(k.w);
 }
 
 static kami_status upload(const void* host, size_t bytes, void** dev) {
@@ -797,7 +823,8 @@ kami_status kami_batched_plan_create(const kami_problem* probs, int64_t count, i
                 WorkItem* d_it = nullptr;
                 st = upload(items.data(), items.size() * sizeof(WorkItem), (void**)&d_it);
                 if (st != KAMI_OK) break;
-                BatchArgs a{d_it, plan->d_counter, (int64_t)items.size(), o.alpha, 0.0, plan->b_kn};
+                BatchArgs a{d_it, plan->d_counter, (int64_t)items.size(), o.alpha, 0.0, plan->b_kn,
+                            queue_shift((int64_t)items.size(), plan->grid)};
                 e = launch_batched(precision, a, plan->grid, s);  // warm-up
                 if (e == hipSuccess) e = hipEventRecord(e0, s);
                 for (int rep = 0; rep < 3 && e == hipSuccess; ++rep)
@@ -845,7 +872,7 @@ kami_status kami_batched_plan_create(const kami_problem* probs, int64_t count, i
         if (e == hipSuccess) e = hipStreamBeginCapture(plan->cap_stream, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
             BatchArgs a{plan->d_items, plan->d_counter, plan->n_items, plan->alpha,
-                        plan->beta, plan->b_kn};
+                        plan->beta, plan->b_kn, queue_shift(plan->n_items, plan->grid)};
             hipError_t le = launch_batched(precision, a, plan->grid, plan->cap_stream);
             e = hipStreamEndCapture(plan->cap_stream, &plan->graph);
             if (le != hipSuccess) e = le;
@@ -868,7 +895,7 @@ kami_status kami_batched_execute(kami_batched_plan* plan, void* stream) {
         e = hipGraphLaunch(plan->exec, s);
     } else {
         BatchArgs a{plan->d_items, plan->d_counter, plan->n_items, plan->alpha,
-                    plan->beta, plan->b_kn};
+                    plan->beta, plan->b_kn, queue_shift(plan->n_items, plan->grid)};
         e = launch_batched(plan->prec, a, plan->grid, s);
     }
     if (e != hipSuccess) return hip_fail(e);

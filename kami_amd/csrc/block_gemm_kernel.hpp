@@ -44,7 +44,7 @@ struct BlockGeom {
     static constexpr int VE = 16 / ELT;  // elements per 16-byte staging chunk
     static constexpr bool A_PRIV = (PN == 1);
     static constexpr int NBUF = (NS == 1) ? 1 : 2;
-    static constexpr int ROWB = KS * ELT + 16;  // padded LDS row: conflict-free 16B reads
+    static constexpr int ROWB = lds_row_bytes(KS * ELT, AT::FRAG_BYTES);  // conflict-free reads
     static constexpr int A_LDS = A_PRIV ? 0 : BM * ROWB;
     static constexpr int B_LDS = BN * ROWB;
     static constexpr int STAGE_LDS = A_LDS + B_LDS;

@@ -31,14 +31,15 @@ struct BlockInstance {
 };
 
 // Largest K stage (a divisor of BK, a multiple of PK * atom K step) whose LDS staging
-// fits `budget` bytes; whole-K staging uses one buffer, partial staging two.
+// fits `budget` bytes, else the smallest such stage; whole-K staging uses one buffer,
+// partial staging two.
 template <class AT, int BM, int BN, int BK, int PM, int PN, int PK>
 constexpr int choose_ks(int budget) {
     constexpr int elt = (int)sizeof(typename AT::in_t);
     int best = 0;
-    for (int ks = BK; ks >= PK * AT::KSTEP; ks /= 2) {
+    for (int ks = BK; ks >= PK * AT::KSTEP; --ks) {
         if (BK % ks != 0 || ks % (PK * AT::KSTEP) != 0) continue;
-        const int rowb = ks * elt + 16;
+        const int rowb = lds_row_bytes(ks * elt, AT::FRAG_BYTES);
         const int stage = ((PN == 1) ? 0 : BM * rowb) + BN * rowb;
         const int nbuf = (ks == BK) ? 1 : 2;
         best = ks;
@@ -70,10 +71,10 @@ template <class AT, int BM, int BN, int BK, int PM, int PN>
 constexpr int choose_ks_pipe(int budget) {
     constexpr int elt = (int)sizeof(typename AT::in_t);
     int best = 0;
-    for (int ks = BK; ks >= AT::KSTEP; ks /= 2) {
+    for (int ks = BK; ks >= AT::KSTEP; --ks) {
         if (BK % ks != 0 || ks % AT::KSTEP != 0) continue;
         best = ks;
-        if (2 * (BM + BN) * (ks * elt + 16) <= budget) return ks;
+        if (2 * (BM + BN) * lds_row_bytes(ks * elt, AT::FRAG_BYTES) <= budget) return ks;
     }
     return best;
 }

@@ -27,6 +27,18 @@ namespace kami {
 
 constexpr int kWave = 64;
 
+// Byte stride of LDS rows that hold `data` bytes each and are read as MFMA fragments:
+// lane l reads `frag` bytes at row (l % 16), offset (l / 16) * frag.  gfx950 serves 32
+// lanes of such a read per pass from a 512-byte bank window, so the read is free of bank
+// conflicts when the stride is frag * s with s twice an odd number (measured for 16-byte
+// fragments: 96-, 160-, 224- and 288-byte rows read at 218-230 B/clk per CU, 144-, 272-
+// and 320-byte rows at 120-126; for 8-byte fragments 80- and 144-byte rows are fastest).
+constexpr int lds_row_bytes(int data, int frag) {
+    int s = (data + frag - 1) / frag;
+    while (s % 4 != 2) ++s;
+    return s * frag;
+}
+
 typedef double d2_t __attribute__((ext_vector_type(2)));
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));

@@ -153,7 +153,7 @@ struct SpNumGeom {
     static constexpr int IM = BS / 16;                 // atom rows (= atom cols)
     static constexpr int KSTEPS = BS / AT::KSTEP;
     static constexpr int ELT = (int)sizeof(typename AT::in_t);
-    static constexpr int ROWB = BS * ELT + 16;          // transposed B row (k contiguous)
+    static constexpr int ROWB = lds_row_bytes(BS * ELT, AT::FRAG_BYTES);  // transposed B row
     static constexpr int LDS = BS * ROWB;
     static constexpr int CHUNKS = BS * BS * ELT / 16;   // 16-byte pieces of a block
     static constexpr int PER_LANE = (CHUNKS + 63) / 64;

@@ -75,7 +75,11 @@ struct SplitGeom {
     static constexpr int TM = BM / PM, TN = BN / PN, IM = TM / 16, JN = TN / 16;
     static constexpr int JH = JN < 2 ? JN : 2;  // column atoms per register group
     static constexpr int KC = 32, NCH = BK / KC;
-    static constexpr int ROWB = KC * 2 + 16;  // 32 digits + 16 B pad: conflict-free b128 reads
+    // 32 digits per row; the conflict-free stride when S digit planes of it fit in LDS,
+    // else 16 bytes of padding
+    static constexpr int ROWB_CF = lds_row_bytes(KC * 2, 16);
+    static constexpr int ROWB =
+        S * (BM + BN) * ROWB_CF + (BM + BN) * 4 <= 160 * 1024 ? ROWB_CF : KC * 2 + 16;
     static constexpr int SA_BYTES = BM * ROWB, SB_BYTES = BN * ROWB;
     static constexpr int PLANE = SA_BYTES + SB_BYTES;
     static constexpr int EXP_OFF = S * PLANE;

@@ -9,7 +9,7 @@
 //     [16 w, 16 w + 16) of the tile and all bs rows of the block row;
 //   * for every stored block (i, kb) the workgroup stages the A block (bs x bs) and the
 //     B slice (64 columns x bs values of K, from B stored N x K or K x N) in LDS, rows
-//     padded by 16 bytes; the next block is loaded into registers while the current one
+//     padded to the conflict-free stride; the next block is loaded into registers while the current one
 //     is multiplied (double-buffered LDS, one barrier per block);
 //   * FP16 / BF16 inputs accumulate in FP32, FP64 in FP64; block rows without stored
 //     blocks still write beta * C.
@@ -24,7 +24,7 @@ struct SpmmGeom {
     using in_t = typename AT::in_t;
     static constexpr int NT = 256, TN = 64;  // C tile: BS rows x TN columns, 4 waves
     static constexpr int ELT = (int)sizeof(in_t), VE = 16 / ELT;
-    static constexpr int ROWB = BS * ELT + 16;
+    static constexpr int ROWB = lds_row_bytes(BS * ELT, AT::FRAG_BYTES);
     static constexpr int A_BYTES = BS * ROWB;
     static constexpr int STAGE = (BS + TN) * ROWB;
     static constexpr int CPR = BS / VE;      // 16-byte pieces per staged row
