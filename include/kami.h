@@ -231,6 +231,38 @@ kami_status kami_bsr_spmm(const kami_bsr* A, const void* B, int32_t n, int64_t l
                           int32_t b_layout, void* C, int64_t ldc, int32_t precision,
                           double alpha, double beta, void* stream);
 
+/* ---------------------------------------------------------------- slicing, generators */
+/* The digit slicing of the split-precision path, standalone: X (rows x k FP64, row-major,
+ * ldx >= k, 0 = packed) is written as
+ *     X(i, j) ~= 2^(e(i, c) - BETA) * sum_{s < slices} d_s(i, j) 2^(-BETA s),   c = j / 32,
+ * with e(i, c) the exponent of the largest |X(i, j)| of the 32-wide chunk c of row i
+ * (every |X| of the chunk below 2^e; 0 for an all-zero chunk) and digits by round to
+ * nearest even, |d_0| <= 2^BETA and |d_s| <= 2^(BETA-1) after.  precision
+ * KAMI_PREC_F64_SPLIT_F16 (BETA = 9, binary16 digits) or KAMI_PREC_F64_SPLIT_BF16
+ * (BETA = 8, bfloat16 digits); 1 <= slices <= 8.  digits (device): `slices` planes of
+ * rows x k 16-bit values, plane s at digits + s * rows * k; exps (device): int32
+ * rows x ceil(k / 32).  The error per element is at most 2^(e - BETA * slices - 1);
+ * finite inputs.  No counterpart in the reference (it has no split-precision path): the
+ * slicing step of kami_block_gemm / kami_split_gemm, for callers that run the slice
+ * products with a GEMM of their own. */
+kami_status kami_split_slices(const double* X, int64_t rows, int64_t k, int64_t ldx,
+                              int32_t precision, int32_t slices, uint16_t* digits,
+                              int32_t* exps, void* stream);
+
+typedef enum kami_generator {
+    KAMI_GEN_RAMP = 0,   /* x[i] = i + offset */
+    KAMI_GEN_RAND_R3 = 1 /* x[i] = rand_r(&seed) % 3 with seed = (unsigned)(i + offset) */
+} kami_generator;
+
+/* Fills dst (device, count values of the precision's input type: double for
+ * KAMI_PREC_F64, binary16 or bfloat16 bits for KAMI_PREC_F16_F32 / _BF16_F32, rounded to
+ * nearest even from the float of the value) with a known-answer input of the reference's
+ * programs, generated on the device instead of on the host and copied: the ramp of
+ * src/block_gemm/KAMI/1d/fp64/block_gemm_1d_double_mma.cu:224-236 and the per-element
+ * rand_r(i) % 3 of src/batched_gemm/KAMI/1d/batched_gemm_1d_double_mma.cu:231-245. */
+kami_status kami_generate(int32_t kind, int32_t precision, void* dst, int64_t count,
+                          int64_t offset, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,14 +10,14 @@ import ctypes
 import os
 
 from . import _abi
-from ._abi import (B_KN, B_NK, PART_1D, PART_2D, PART_3D, PART_AUTO, PREC_BF16_F32,
-                   PREC_F16_F32, PREC_F64, PREC_F64_SPLIT_BF16, PREC_F64_SPLIT_F16,
-                   KamiError, check, lib)
+from ._abi import (B_KN, B_NK, GEN_RAMP, GEN_RAND_R3, PART_1D, PART_2D, PART_3D, PART_AUTO,
+                   PREC_BF16_F32, PREC_F16_F32, PREC_F64, PREC_F64_SPLIT_BF16,
+                   PREC_F64_SPLIT_F16, KamiError, check, lib)
 
 __all__ = [
     "block_gemm", "query", "make_desc", "KamiError", "lib", "version", "device_ok",
     "PREC_F64", "PREC_F16_F32", "PREC_BF16_F32", "PREC_F64_SPLIT_F16", "PREC_F64_SPLIT_BF16",
-    "PART_AUTO", "PART_1D", "PART_2D", "PART_3D", "B_NK", "B_KN",
+    "PART_AUTO", "PART_1D", "PART_2D", "PART_3D", "B_NK", "B_KN", "GEN_RAMP", "GEN_RAND_R3",
 ]
 
 _PART_NAMES = {"auto": PART_AUTO, "1d": PART_1D, "2d": PART_2D, "3d": PART_3D}
@@ -148,6 +148,44 @@ def split_gemm(A, B, C=None, *, precision=PREC_F64_SPLIT_F16, slices=0, alpha=1.
 
 
 __all__ += ["split_gemm"]
+
+
+def split_slices(X, *, precision=PREC_F64_SPLIT_F16, slices=6, stream=None):
+    """Digit slices of an FP64 matrix as the split-precision path forms them
+    (kami_split_slices): returns (digits, exps) with digits a uint16 tensor of shape
+    (slices, rows, k) holding binary16 (FP16 precision) or bfloat16 (BF16) bits and exps
+    an int32 tensor (rows, ceil(k / 32)), so that
+    X[i, j] ~= sum_s digits[s, i, j] * 2^(exps[i, j // 32] - BETA * (s + 1)).
+    X: (rows, k) float64 device tensor whose rows are contiguous."""
+    import torch
+    if X.dim() != 2 or X.dtype != torch.float64 or X.stride(1) != 1:
+        raise ValueError("split_slices expects a (rows, k) float64 tensor with unit column stride")
+    rows, k = X.shape
+    digits = torch.empty((slices, rows, k), dtype=torch.int16, device=X.device)
+    exps = torch.empty((rows, (k + 31) // 32), dtype=torch.int32, device=X.device)
+    check(lib().kami_split_slices(ctypes.c_void_p(X.data_ptr()), rows, k, X.stride(0),
+                                  precision, slices, ctypes.c_void_p(digits.data_ptr()),
+                                  ctypes.c_void_p(exps.data_ptr()), _stream_ptr(stream)),
+          f"kami_split_slices({rows}x{k}, prec={precision}, slices={slices})")
+    return digits.view(torch.uint16), exps
+
+
+def generate(kind, count, *, precision=PREC_F64, offset=0, device="cuda", stream=None):
+    """A known-answer input of the reference's programs generated on the device
+    (kami_generate): kind GEN_RAMP (x[i] = i + offset) or GEN_RAND_R3 (rand_r(i + offset)
+    % 3).  Returns a float64 tensor, or float16 / bfloat16 for the FP16 / BF16 precisions."""
+    import torch
+    dt = {PREC_F64: torch.float64, PREC_F16_F32: torch.float16,
+          PREC_BF16_F32: torch.bfloat16}.get(precision)
+    if dt is None:
+        raise KamiError(_abi.ENOTSUP, f"generate: precision {precision}")
+    out = torch.empty(count, dtype=dt, device=device)
+    check(lib().kami_generate(kind, precision, ctypes.c_void_p(out.data_ptr()), count, offset,
+                              _stream_ptr(stream)), f"kami_generate(kind={kind})")
+    return out
+
+
+__all__ += ["split_slices", "generate"]
 
 
 class BatchedPlan:

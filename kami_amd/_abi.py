@@ -28,6 +28,8 @@ PREC_F64, PREC_F16_F32, PREC_BF16_F32, PREC_F64_SPLIT_F16, PREC_F64_SPLIT_BF16 =
 PART_AUTO, PART_1D, PART_2D, PART_3D = range(4)
 # kami_b_layout
 B_NK, B_KN = 0, 1
+# kami_generator
+GEN_RAMP, GEN_RAND_R3 = 0, 1
 
 
 class GemmDesc(ctypes.Structure):
@@ -102,6 +104,9 @@ _PROTOTYPES = {
     "kami_tune_clear": (c_int32, []),
     "kami_bsr_spmm": (c_int32, [ctypes.POINTER(Bsr), c_void_p, c_int32, c_int64, c_int32,
                                 c_void_p, c_int64, c_int32, c_double, c_double, c_void_p]),
+    "kami_split_slices": (c_int32, [c_void_p, c_int64, c_int64, c_int64, c_int32, c_int32,
+                                    c_void_p, c_void_p, c_void_p]),
+    "kami_generate": (c_int32, [c_int32, c_int32, c_void_p, c_int64, c_int64, c_void_p]),
 }
 
 

@@ -9,6 +9,9 @@
 // of a problem so ring slots and prefetch targets are compile-time (NS % PD == 0).
 #pragma once
 
+#include <algorithm>
+#include <atomic>
+
 #include "block_gemm_kernel.hpp"
 
 namespace kami {
@@ -153,23 +156,40 @@ block_gemm_pipe_kernel(BlockArgs args, int64_t count) {
     }
 }
 
+// Workgroups of `kernel` (`threads` per workgroup) that the current device holds at once:
+// CUs x occupancy (at least one per CU).  Cached per device in `cache`: the value depends
+// only on the kernel and the device, so concurrent first calls store the same value.
+constexpr int kMaxCachedDevices = 64;
+template <class F>
+hipError_t resident_workgroups(F kernel, int threads,
+                               std::atomic<int> (&cache)[kMaxCachedDevices], int* out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const bool cached = dev >= 0 && dev < kMaxCachedDevices;
+    int v = cached ? cache[dev].load(std::memory_order_relaxed) : 0;
+    if (v == 0) {
+        int n = 0, occ = 0;
+        e = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, threads, 0);
+        if (e != hipSuccess) return e;
+        v = std::max(1, n) * std::max(1, occ);
+        if (cached) cache[dev].store(v, std::memory_order_relaxed);
+    }
+    *out = v;
+    return hipSuccess;
+}
+
 // Persistent grid: every CU filled to the kernel's occupancy, never more workgroups than
 // problems.
 template <class AT, class G, int PD>
 hipError_t launch_block_gemm_pipe(const BlockArgs& args, int64_t count, hipStream_t stream) {
-    static int per_cu = -1, cus = 0;
-    if (per_cu < 0) {
-        int dev = 0, n = 0, occ = 0;
-        hipError_t e = hipGetDevice(&dev);
-        if (e == hipSuccess) e = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
-        if (e == hipSuccess)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &occ, block_gemm_pipe_kernel<AT, G, PD>, G::NT, 0);
-        if (e != hipSuccess) return e;
-        cus = n;
-        per_cu = occ > 0 ? occ : 1;
-    }
-    const int64_t grid = std::min<int64_t>(count, (int64_t)cus * per_cu);
+    static std::atomic<int> cache[kMaxCachedDevices];
+    int slots = 0;
+    const hipError_t e =
+        resident_workgroups(block_gemm_pipe_kernel<AT, G, PD>, G::NT, cache, &slots);
+    if (e != hipSuccess) return e;
+    const int64_t grid = std::min<int64_t>(count, (int64_t)slots);
     hipLaunchKernelGGL((block_gemm_pipe_kernel<AT, G, PD>), dim3((unsigned)grid), dim3(G::NT), 0,
                        stream, args, count);
     return hipGetLastError();

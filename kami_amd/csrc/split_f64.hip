@@ -507,18 +507,11 @@ hipError_t launch_split(const SplitArgs& a, int64_t count, hipStream_t s) {
 // than problems.
 template <class SAT, class G>
 hipError_t launch_split_pipe(const SplitArgs& a, int64_t count, hipStream_t s) {
-    static int per_cu = -1, cus = 0;
-    if (per_cu < 0) {
-        int dev = 0, n = 0, occ = 0;
-        hipError_t e = hipGetDevice(&dev);
-        if (e == hipSuccess) e = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
-        if (e == hipSuccess)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, split_pipe_kernel<SAT, G>, G::NT, 0);
-        if (e != hipSuccess) return e;
-        cus = n;
-        per_cu = occ > 0 ? occ : 1;
-    }
-    const int64_t grid = std::min<int64_t>(count, (int64_t)cus * per_cu);
+    static std::atomic<int> cache[kMaxCachedDevices];
+    int slots = 0;
+    const hipError_t e = resident_workgroups(split_pipe_kernel<SAT, G>, G::NT, cache, &slots);
+    if (e != hipSuccess) return e;
+    const int64_t grid = std::min<int64_t>(count, (int64_t)slots);
     hipLaunchKernelGGL((split_pipe_kernel<SAT, G>), dim3((unsigned)grid), dim3(G::NT), 0, s, a,
                        count);
     return hipGetLastError();

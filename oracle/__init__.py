@@ -142,6 +142,40 @@ def randr_mod3(n):
     return x
 
 
+def randr_mod3_from(n, offset):
+    """rand_r(seed = offset + i) % 3 for i < n (the per-element generator above, from an
+    offset)."""
+    return randr_mod3(n + offset)[offset:]
+
+
+def split_slices(X, beta, slices, bf16=False):
+    """Digit slices of the split-precision path, restated from their definition: per row
+    and 32-wide chunk the exponent e of the largest |x| (|x| < 2^e, 0 for an all-zero
+    chunk); r = x 2^(beta - e); `slices` times d = rint(r) (ties to even), r = (r - d) 2^beta.
+    Returns (digits uint16 (slices, rows, k) as binary16 or bfloat16 bits, exps int32
+    (rows, ceil(k / 32))).  Pinned in tests/test_oracle.py by reconstruction within
+    2^(e - beta slices - 1), the digit bounds and exactness on short-mantissa inputs."""
+    X = np.asarray(X, dtype=np.float64)
+    rows, k = X.shape
+    nch = (k + 31) // 32
+    pad = np.zeros((rows, nch * 32))
+    pad[:, :k] = X
+    ex = np.frexp(pad)[1]
+    ex = np.where(pad == 0, -4096, ex)
+    e = ex.reshape(rows, nch, 32).max(axis=2)
+    e = np.where(e == -4096, 0, e).astype(np.int32)
+    r = np.ldexp(X, beta - np.repeat(e, 32, axis=1)[:, :k])
+    digits = np.empty((slices, rows, k), dtype=np.uint16)
+    for s in range(slices):
+        q = np.rint(r)
+        if bf16:
+            digits[s] = (q.astype(np.float32).view(np.uint32) >> 16).astype(np.uint16)
+        else:
+            digits[s] = q.astype(np.float16).view(np.uint16)
+        r = (r - q) * float(2 ** beta)
+    return digits, e
+
+
 def half_to_double(h):
     return lib().oracle_half_to_double(int(h))
 

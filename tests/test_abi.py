@@ -197,3 +197,35 @@ def test_split_gemm_validation_without_device():
     assert call(0, 128, 64) == _abi.EINVAL
     for shape in [(256, 384, 96), (192, 64, 32), (4096, 4096, 4096)]:
         assert call(*shape) == _abi.ENODEV                        # valid; no GPU here
+
+
+def test_split_slices_and_generate_validation_without_device():
+    lib = K.lib()
+    X, D, E = 1 << 20, 1 << 21, 1 << 22
+
+    def sl(x=X, rows=4, k=64, ldx=0, prec=K.PREC_F64_SPLIT_F16, slices=6, d=D, e=E):
+        return lib.kami_split_slices(x, rows, k, ldx, prec, slices, d, e, None)
+
+    assert sl(x=None) == _abi.EINVAL
+    assert sl(d=None) == _abi.EINVAL
+    assert sl(rows=-1) == _abi.EINVAL
+    assert sl(slices=0) == _abi.EINVAL
+    assert sl(slices=9) == _abi.EINVAL
+    assert sl(ldx=32) == _abi.EINVAL           # ldx < k
+    assert sl(prec=K.PREC_F64) == _abi.ENOTSUP
+    assert sl(rows=0) == _abi.OK               # nothing to do
+    if lib.kami_device_check() == _abi.ENODEV:
+        assert sl() == _abi.ENODEV
+
+    def gen(kind=K.GEN_RAMP, prec=K.PREC_F64, dst=X, count=16, offset=0):
+        return lib.kami_generate(kind, prec, dst, count, offset, None)
+
+    assert gen(kind=5) == _abi.EINVAL
+    assert gen(dst=None) == _abi.EINVAL
+    assert gen(count=-1) == _abi.EINVAL
+    assert gen(offset=-3) == _abi.EINVAL
+    assert gen(dst=X + 4) == _abi.EINVAL       # doubles need 8-byte alignment
+    assert gen(prec=K.PREC_F64_SPLIT_F16) == _abi.ENOTSUP
+    assert gen(count=0) == _abi.OK
+    if lib.kami_device_check() == _abi.ENODEV:
+        assert gen() == _abi.ENODEV

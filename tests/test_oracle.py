@@ -294,3 +294,54 @@ def test_golden_fixtures_match_oracle():
         C = oracle.gemm_f64(A, A)
         for (i, j, v) in entry["samples"]:
             assert C[0, i, j] == v
+
+
+@pytest.mark.parametrize("beta,slices,bf16", [(9, 6, False), (8, 7, True), (9, 2, False)])
+def test_split_slices_reconstruct_within_bound(beta, slices, bf16):
+    """The digits and exponents give X back within 2^(e - beta slices - 1) per element,
+    each chunk exponent bounds its chunk (|x| < 2^e, and some |x| >= 2^(e-1)), and the
+    digits obey |d_0| <= 2^beta, |d_s| <= 2^(beta-1) -- checked from the outputs alone."""
+    rng = np.random.default_rng(7)
+    rows, k = 9, 70  # a partial last chunk
+    X = np.sign(rng.standard_normal((rows, k))) * 10.0 ** rng.uniform(-12, 12, (rows, k))
+    X[3] = 0.0                       # all-zero row
+    X[4, :32] = 0.0                  # one all-zero chunk
+    X[5, 7] = -0.0
+    X[6, :5] = [2.0 ** -1074, 3 * 2.0 ** -1070, 1e300, -1e-300, 0.5]
+    digits, e = oracle.split_slices(X, beta, slices, bf16)
+    if bf16:
+        vals = (digits.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    else:
+        vals = digits.view(np.float16).astype(np.float64)
+    assert np.all(vals == np.rint(vals))
+    assert np.all(np.abs(vals[0]) <= 2.0 ** beta) and np.all(np.abs(vals[1:]) <= 2.0 ** (beta - 1))
+    ec = np.repeat(e, 32, axis=1)[:, :k]
+    for i in range(rows):  # exact rational arithmetic: no rounding in the check itself
+        for j in range(k):
+            recon = sum(Fraction(vals[s, i, j]) * Fraction(2) ** int(ec[i, j] - beta * (s + 1))
+                        for s in range(slices))
+            assert abs(recon - Fraction(X[i, j])) <= Fraction(2) ** int(ec[i, j] - beta * slices - 1)
+    for i in range(rows):
+        for c in range((k + 31) // 32):
+            chunk = np.abs(X[i, 32 * c:32 * c + 32])
+            if chunk.max() == 0:
+                assert e[i, c] == 0 and np.all(vals[:, i, 32 * c:32 * c + 32] == 0)
+            else:
+                assert chunk.max() < 2.0 ** e[i, c] and chunk.max() >= 2.0 ** (e[i, c] - 1)
+
+
+def test_split_slices_exact_on_short_mantissas():
+    """Values with at most beta * slices significant bits below the chunk maximum come
+    back exactly; a one-slice split of powers of two is the power itself."""
+    X = np.array([[3.0, -5.0, 255.0, 1.0, 0.25, -0.125] + [0.0] * 26])
+    d, e = oracle.split_slices(X, 9, 2)
+    vals = d.view(np.float16).astype(np.float64)
+    recon = sum(np.ldexp(vals[s], e[0, 0] - 9 * (s + 1)) for s in range(2))
+    assert np.array_equal(recon, X) and e[0, 0] == 8
+    P = np.array([[2.0 ** 40, -2.0 ** 40]])
+    d1, e1 = oracle.split_slices(P, 9, 1)
+    assert e1[0, 0] == 41 and list(d1[0, 0].view(np.float16).astype(float)) == [256.0, -256.0]
+
+
+def test_randr_mod3_from_offset_is_a_window():
+    assert np.array_equal(oracle.randr_mod3_from(100, 37), oracle.randr_mod3(137)[37:])
