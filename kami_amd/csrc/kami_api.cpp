@@ -222,7 +222,6 @@ const BlockInstance* resolve_block(const kami_gemm_desc* d, kami_status* st) {
         }
         if (pipe == 0) pipe = ppipe;
     }
-    if (pipe == 0) pipe = 1;
     if (pipe == 2 && !pipe_allowed(d)) {
         if (d->pipeline == 2) {
             *st = KAMI_ENOTSUP;
@@ -232,6 +231,19 @@ const BlockInstance* resolve_block(const kami_gemm_desc* d, kami_status* st) {
     }
     const bool ranks_defaulted = ranks == 0;
     if (ranks_defaulted) ranks = default_ranks(d->precision, part, d->m, d->n, d->k);
+    if (pipe == 0) {
+        // a partition (and wave count) given without a form: for FP64 the persistent form
+        // where one is compiled for it (measured faster at every such shape: 64^3 x 1024
+        // 2D P=2 20.3 us vs 24.6 us, 32^3 x 4096 15.4 vs 17.0 us, 128^3 x 4096 48 vs 37
+        // TFLOPS); FP16 / BF16 stream at the HBM rate either way and one workgroup per
+        // problem is as fast or faster (128^3 x 8192: 194-199 us vs 206-214 us)
+        pipe = 1;
+        if (pipe_allowed(d) && d->precision == KAMI_PREC_F64)
+            for (const auto& in : registry())
+                if (in.prec == d->precision && in.m == d->m && in.n == d->n && in.k == d->k &&
+                    in.part == part && in.ranks == ranks && in.pipe == 2)
+                    pipe = 2;
+    }
     for (const auto& in : registry()) {
         if (in.prec == d->precision && in.m == d->m && in.n == d->n && in.k == d->k &&
             in.part == part && in.ranks == ranks && in.pipe == pipe) {
