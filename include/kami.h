@@ -195,7 +195,8 @@ kami_status kami_batched_plan_destroy(kami_batched_plan* plan);
 /* Workspace for symbolic/fill of C = A * B, A with mb block rows, B with nb block cols. */
 kami_status kami_bsr_spgemm_workspace_size(int32_t mb, int32_t nb, size_t* bytes);
 /* rowptrC (mb+1, device) receives the exclusive scan of per-row output block counts;
- * *nnzbC receives rowptrC[mb] (copied to the host: this call synchronises `stream`). */
+ * *nnzbC receives rowptrC[mb] (copied to the host: this call synchronises `stream`).
+ * KAMI_ENOTSUP when C would hold 2^31 - 1 blocks or more (int32 row pointers). */
 kami_status kami_bsr_spgemm_symbolic(const kami_bsr* A, const kami_bsr* B, int32_t* rowptrC,
                                      int64_t* nnzbC, void* ws, size_t ws_bytes, void* stream);
 /* colidxC (nnzbC, device) receives the ascending output block columns of every row. */

@@ -69,9 +69,13 @@ spgemm_count_kernel(int mb, int nwords, const int32_t* rpA, const int32_t* ciA,
 
 // In-place exclusive scan of x[0..n-2] into x[0..n-1] (x[n-1] receives the total;
 // its input value is ignored).  One workgroup of 1024 threads.
+// Exclusive scan of the n - 1 per-row counts in x into x[0 .. n-1].  The running total is
+// kept in 64 bits and every prefix is stored saturated at INT32_MAX, so an output with
+// 2^31 - 1 or more blocks (which int32 row pointers cannot hold) shows as x[n-1] ==
+// INT32_MAX instead of wrapping.
 __global__ void __launch_bounds__(1024) spgemm_scan_kernel(int32_t* x, int n) {
     __shared__ int s_wave[16];
-    __shared__ int s_carry;
+    __shared__ long long s_carry;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid == 0) s_carry = 0;
     __syncthreads();
@@ -96,9 +100,9 @@ __global__ void __launch_bounds__(1024) spgemm_scan_kernel(int32_t* x, int n) {
             if (lane < 16) s_wave[lane] = w;  // inclusive wave totals
         }
         __syncthreads();
-        const int carry = s_carry;
-        const int before = (wave > 0 ? s_wave[wave - 1] : 0) + incl - v;
-        if (i < n) x[i] = carry + before;
+        const long long carry = s_carry;
+        const int before = (wave > 0 ? s_wave[wave - 1] : 0) + incl - v;  // < 2^30 per chunk
+        if (i < n) x[i] = (int32_t)min(carry + before, (long long)INT32_MAX);
         __syncthreads();
         if (tid == 0) s_carry = carry + s_wave[15];
         __syncthreads();
@@ -326,6 +330,7 @@ kami_status kami_bsr_spgemm_symbolic(const kami_bsr* A, const kami_bsr* B, int32
     e = hipMemcpyAsync(&total, rowptrC + A->mb, sizeof(int32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(e);
+    if (total == INT32_MAX) return KAMI_ENOTSUP;  // more blocks than int32 row pointers hold
     *nnzbC = total;
     return KAMI_OK;
 }
