@@ -548,8 +548,11 @@ batched_persistent_kernel(BatchArgs args) {
                     st.sv0 = -1;
                 }
             }
-            // then stages x + 1 and x + 2 into their slots
-            while (st.loaded < st.x + 3 && st.ct.valid) load_next_into_slot<AT, KN>(st, q);
+            // then stages x + 1 and x + 2 into their slots, as far as the fetched tiles
+            // reach: a cursor that blocks again (the batch just fetched holds fewer
+            // stages than that) stays blocked until the next tile boundary repairs it
+            while (st.loaded < st.x + 3 && st.ct.valid && !st.blocked)
+                load_next_into_slot<AT, KN>(st, q);
             __syncthreads();
         }
     }

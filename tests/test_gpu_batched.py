@@ -147,3 +147,22 @@ def test_autotuned_plan_is_exact_and_reports_buckets(dtype, dev):
         torch.cuda.synchronize()
     for (At, Bt, Ct), ref in zip(probs, reference(host)):
         np.testing.assert_array_equal(Ct.double().cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float16], ids=["f64", "f16"])
+@pytest.mark.parametrize("autotune", [False, True], ids=["priors", "tuned"])
+def test_short_queue_of_one_stage_tiles(dtype, autotune, dev):
+    """A short queue (few tiles per workgroup, so two entries per pop) of mixed K where
+    most tiles have a single K stage: the cursor runs past the fetched tiles again right
+    after a boundary repair, which must leave it blocked until the next boundary (a
+    regression test: the repair used to loop on a blocked cursor)."""
+    rng = np.random.default_rng(2024)
+    sizes = [(int(rng.choice([16, 32, 48, 64])), int(rng.choice([16, 32, 64])),
+              int(rng.choice([16, 16, 16, 16, 32, 64, 256]))) for _ in range(1500)]
+    host, probs = make_problems(sizes, dtype, dev, rng)
+    with K.BatchedPlan(probs, autotune=autotune, use_graph=False) as plan:
+        assert plan.info()["work_items"] < 6 * plan.info()["workgroups"]  # two per pop
+        plan.execute()
+        torch.cuda.synchronize()
+    for (A, B, C), ref in zip(probs, reference(host)):
+        np.testing.assert_array_equal(C.double().cpu().numpy(), ref)
