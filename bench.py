@@ -218,7 +218,8 @@ def main():
         from kami_amd import workloads as W
         others = {}
         for name, fn in [("w1_f64_64cubed_x1024", W.run_w1), ("w2_f16_block_gemm", W.run_w2),
-                         ("w4_batched_uneven_x20000", W.run_w4), ("w5_block_spgemm", W.run_w5)]:
+                         ("w4_batched_uneven_x20000", W.run_w4), ("w5_block_spgemm", W.run_w5),
+                         ("w6_block_spmm", W.run_w6)]:
             t0 = time.perf_counter()
             try:
                 others[name] = fn()

@@ -215,3 +215,39 @@ def run_w5(steps=10, warmup=2):
             "nnzb_A": int(len(ci)), "nnzb_C": int(nnz), "nnzCub": nnzcub,
             "symbolic_ms": ms_sym, "numeric_ms": ms_num, "numeric_tflops": flops / ms_num / 1e9,
             "numeric_hbm_gbs": byts / ms_num / 1e6}
+
+
+# ------------------------------------------------------------------ w6
+
+def run_w6(steps=20, warmup=3, n=256):
+    """Block SpMM C = A B^T with the w5 matrix (8192^2, 32x32 blocks, 2 % of blocks, FP16
+    N(0,1) values) and a dense B of n rows x 8192 (the reference's N x K layout), FP16
+    inputs with FP32 C and FP64 inputs and C."""
+    torch = _torch()
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(SEED + 4)
+    mb, bs = 256, 32
+    rp, ci, vals = spgemm_input(rng, mb, bs)
+    nnzb, kdim = len(ci), mb * bs
+    Bh = rng.standard_normal((n, kdim)).astype(np.float16)
+    Bd = Bh.astype(np.float64)
+    ref = np.zeros((kdim, n))  # block by block in FP64
+    for i in range(mb):
+        for q in range(rp[i], rp[i + 1]):
+            ref[i * bs:(i + 1) * bs] += vals[q].astype(np.float64) @ Bd[:, ci[q] * bs:(ci[q] + 1) * bs].T
+    out = {"workload": f"BSR {kdim}^2 (32x32 blocks, 2% density) x dense {n} x {kdim}",
+           "nnzb_A": int(nnzb), "n": n}
+    flops = 2.0 * nnzb * bs * bs * n
+    for name, prec, dt in [("f16", K.PREC_F16_F32, torch.float16), ("f64", K.PREC_F64, torch.float64)]:
+        A = K.BsrMatrix(mb, mb, bs, torch.from_numpy(rp).to(dev), torch.from_numpy(ci).to(dev),
+                        torch.from_numpy(vals.astype(np.float64)).to(dev).to(dt))
+        B = torch.from_numpy(Bh.astype(np.float64)).to(dev).to(dt)
+        C = K.bsr_spmm(A, B, precision=prec)
+        ms = time_ms(lambda: K.bsr_spmm(A, B, C, precision=prec), steps, warmup)
+        err = float(np.max(np.abs(C.double().cpu().numpy() - ref)) / np.max(np.abs(ref)))
+        ein, eout = (2, 4) if name == "f16" else (8, 8)
+        byts = ein * (nnzb * bs * bs + n * kdim) + eout * kdim * n  # A, B read once, C written
+        out[name] = {"ms": ms, "tflops": flops / ms / 1e9, "hbm_gbs": byts / ms / 1e6,
+                     "max_rel_err_vs_fp64": err}
+    return out
+
