@@ -166,3 +166,35 @@ def test_short_queue_of_one_stage_tiles(dtype, autotune, dev):
         torch.cuda.synchronize()
     for (A, B, C), ref in zip(probs, reference(host)):
         np.testing.assert_array_equal(C.double().cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float16], ids=["f64", "f16"])
+def test_large_and_tiny_problems_in_one_plan(dtype, dev):
+    """Problems far beyond 256 (hundreds of K stages, tens of tiles each) next to 16^3
+    ones and ragged edges in one plan; integer inputs keep every result exact (FP16:
+    |sums| <= 4 K stay exact in FP32)."""
+    rng = np.random.default_rng(77)
+    sizes = [(1024, 768, 2048), (528, 1040, 1200), (16, 16, 16), (48, 1008, 4096),
+             (1008, 16, 512)] + [(16, 16, 16)] * 40 + [(272, 304, 336)] * 3
+    host, probs = make_problems(sizes, dtype, dev, rng)
+    with K.BatchedPlan(probs, autotune=False, use_graph=True) as plan:
+        plan.execute()
+        torch.cuda.synchronize()
+    for (A, B, C), ref in zip(probs, reference(host)):
+        np.testing.assert_array_equal(C.double().cpu().numpy(), ref)
+
+
+def test_hundred_thousand_tiny_problems(dev):
+    """A long queue of the smallest problems (16^3, one tile and one K stage each)."""
+    count, s = 100000, 16
+    rng = np.random.default_rng(78)
+    A = torch.from_numpy(rng.integers(-2, 3, (count, s, s)).astype(np.float64)).to(dev)
+    B = torch.from_numpy(rng.integers(-2, 3, (count, s, s)).astype(np.float64)).to(dev)
+    C = torch.full((count, s, s), float("nan"), dtype=torch.float64, device=dev)
+    probs = [(A[i], B[i], C[i]) for i in range(count)]
+    with K.BatchedPlan(probs, precision=K.PREC_F64, autotune=False, use_graph=False) as plan:
+        assert plan.info()["work_items"] == count
+        plan.execute()
+        torch.cuda.synchronize()
+    ref = np.einsum("pik,pjk->pij", A.cpu().numpy(), B.cpu().numpy())
+    np.testing.assert_array_equal(C.cpu().numpy(), ref)
