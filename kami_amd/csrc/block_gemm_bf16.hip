@@ -37,6 +37,8 @@ void register_bf16_instances(std::vector<BlockInstance>& v) {
     // 256^3
     add_instance<A, F, 256, 256, 256, 8, 1, 1>(v, 1, 8);
     add_instance<A, F, 256, 256, 256, 16, 1, 1>(v, 1, 16);
+    add_resident_instance<A, F, 256, 256, 256, 8>(v, 8);
+    add_resident_instance<A, F, 256, 256, 256, 16>(v, 16);
     add_instance<A, F, 256, 256, 256, 4, 4, 1>(v, 2, 4);
     // low rank 256 x 256 x 16 and x 32
     add_instance<H, F, 256, 256, 16, 8, 1, 1>(v, 1, 8);
@@ -52,6 +54,9 @@ void register_bf16_instances(std::vector<BlockInstance>& v) {
     add_instance<H, F, 16, 16, 16, 1, 1, 1>(v, 2, 1);
     add_instance<H, F, 16, 16, 16, 1, 1, 1>(v, 3, 1);
     add_instance<A, F, 192, 192, 192, 4, 4, 1>(v, 2, 4);
+    add_resident_instance<A, F, 192, 192, 192, 4, 4>(v, 4);
+    // 1D, 12 waves of 16 rows: all of K fits one stage, so this form is resident as is
+    add_instance<A, F, 192, 192, 192, 12, 1, 1, 2>(v, 1, 12);
     // low-rank M = N in {32, 64, 128, 192}, K in {16, 32}
     // (scripts/fp16_block_lowrank_GH200.sh:20-27)
     add_instance<H, F, 32, 32, 16, 1, 1, 1>(v, 1, 1);

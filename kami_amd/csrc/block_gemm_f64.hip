@@ -29,6 +29,7 @@ void register_f64_instances(std::vector<BlockInstance>& v) {
     // 128^3
     add_instance<A, F, 128, 128, 128, 8, 1, 1>(v, 1, 8);
     add_instance<A, F, 128, 128, 128, 4, 1, 1>(v, 1, 4);
+    add_resident_instance<A, F, 128, 128, 128, 8>(v, 8);
     add_instance<A, F, 128, 128, 128, 2, 2, 1>(v, 2, 2);
     add_instance<A, F, 128, 128, 128, 4, 4, 1>(v, 2, 4);
     add_instance<A, F, 128, 128, 128, 2, 2, 2>(v, 3, 2);

@@ -79,7 +79,10 @@ struct BlockArgs {
     int32_t b_kn;             // 1: B stored K x N
 };
 
-template <class AT, class G>
+// KFENCE > 0: a scheduling fence after every KFENCE K steps of a stage, so the
+// compiler does not hoist the B fragment loads of a whole long stage ahead of the MFMAs
+// (used by the resident forms, whose single stage covers all of K).
+template <class AT, class G, int KFENCE = 0>
 __global__ void __launch_bounds__(G::NT)
 block_gemm_kernel(BlockArgs args) {
     using in_t = typename AT::in_t;
@@ -253,6 +256,8 @@ block_gemm_kernel(BlockArgs args) {
 #pragma unroll
                         for (int im = 0; im < G::IM; ++im) AT::mma_sub(acc[im][jn], af[im], bf[jn], e);
             }
+            if constexpr (KFENCE > 0)
+                if ((ks + 1) % KFENCE == 0) __builtin_amdgcn_sched_barrier(0);
         }
     };
 
@@ -355,9 +360,10 @@ block_gemm_kernel(BlockArgs args) {
         }
 }
 
-template <class AT, class G>
+template <class AT, class G, int KFENCE = 0>
 hipError_t launch_block_gemm(const BlockArgs& args, int64_t count, hipStream_t stream) {
-    hipLaunchKernelGGL((block_gemm_kernel<AT, G>), dim3((unsigned)count), dim3(G::NT), 0, stream, args);
+    hipLaunchKernelGGL((block_gemm_kernel<AT, G, KFENCE>), dim3((unsigned)count), dim3(G::NT), 0,
+                       stream, args);
     return hipGetLastError();
 }
 

@@ -28,6 +28,7 @@ struct BlockInstance {
     block_launch_fn launch;
     int pipe;       // 1: one workgroup per problem; 2: persistent, deep prefetch
     int prefetch;   // K stages loaded ahead of the MFMAs
+    int resident = 0;  // 1: one LDS stage holds all of K (chosen for iters > 1)
 };
 
 // Largest K stage (a divisor of BK, a multiple of PK * atom K step) whose LDS staging
@@ -50,12 +51,27 @@ constexpr int choose_ks(int budget) {
 
 constexpr int kStageBudget = 80 * 1024;
 
-template <class AT, int PREC, int BM, int BN, int BK, int PM, int PN, int PK>
+// KFENCE: see block_gemm_kernel (for stages of many K steps).
+template <class AT, int PREC, int BM, int BN, int BK, int PM, int PN, int PK, int KFENCE = 0>
 void add_instance(std::vector<BlockInstance>& v, int part, int ranks) {
     constexpr int KS = choose_ks<AT, BM, BN, BK, PM, PN, PK>(kStageBudget);
     using G = BlockGeom<AT, BM, BN, BK, PM, PN, PK, KS>;
     v.push_back(BlockInstance{PREC, BM, BN, BK, part, ranks, PM, PN, PK, KS, G::NT, G::LDS,
-                              AT::KSTEP, &launch_block_gemm<AT, G>, 1, 1});
+                              AT::KSTEP, &launch_block_gemm<AT, G, KFENCE>, 1, 1});
+}
+
+// Resident-repeat form of a 1D or 2D instance: one LDS stage holds the whole K of B (and
+// of A in 2D; in 1D each wave keeps its rows of A in registers), so the repeated
+// products of iters > 1 (the reference's NUM_ITER loop,
+// src/block_gemm/KAMI/1d/fp64/block_gemm_1d_double_mma.cu:140) read nothing from memory
+// after the first; the streaming form of the same key stages K in chunks instead.
+// Chosen by the dispatcher when iters > 1.
+template <class AT, int PREC, int BM, int BN, int BK, int PM, int PN = 1>
+void add_resident_instance(std::vector<BlockInstance>& v, int ranks) {
+    using G = BlockGeom<AT, BM, BN, BK, PM, PN, 1, BK>;
+    static_assert(G::NS == 1, "whole K in one stage");
+    v.push_back(BlockInstance{PREC, BM, BN, BK, PN == 1 ? 1 : 2, ranks, PM, PN, 1, BK, G::NT,
+                              G::LDS, AT::KSTEP, &launch_block_gemm<AT, G, 2>, 1, 1, 1});
 }
 
 // Largest prefetch depth <= want that divides the stage count.

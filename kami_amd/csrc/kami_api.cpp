@@ -231,6 +231,18 @@ const BlockInstance* resolve_block(const kami_gemm_desc* d, kami_status* st) {
     }
     const bool ranks_defaulted = ranks == 0;
     if (ranks_defaulted) ranks = default_ranks(d->precision, part, d->m, d->n, d->k);
+    // repeated products (iters > 1) use a key's resident form (all of K in one LDS stage)
+    // where one is compiled; a single product uses the streaming form
+    auto same_key = [&](const BlockInstance& in, int want_pipe) {
+        return in.prec == d->precision && in.m == d->m && in.n == d->n && in.k == d->k &&
+               in.part == part && in.ranks == ranks && in.pipe == want_pipe;
+    };
+    if (d->iters > 1 && (pipe == 0 || pipe == 1))
+        for (const auto& in : registry())
+            if (in.resident && same_key(in, 1)) {
+                *st = KAMI_OK;
+                return &in;
+            }
     if (pipe == 0) {
         // a partition (and wave count) given without a form: for FP64 the persistent form
         // where one is compiled for it (measured faster at every such shape: 64^3 x 1024
@@ -240,13 +252,10 @@ const BlockInstance* resolve_block(const kami_gemm_desc* d, kami_status* st) {
         pipe = 1;
         if (pipe_allowed(d) && d->precision == KAMI_PREC_F64)
             for (const auto& in : registry())
-                if (in.prec == d->precision && in.m == d->m && in.n == d->n && in.k == d->k &&
-                    in.part == part && in.ranks == ranks && in.pipe == 2)
-                    pipe = 2;
+                if (same_key(in, 2)) pipe = 2;
     }
     for (const auto& in : registry()) {
-        if (in.prec == d->precision && in.m == d->m && in.n == d->n && in.k == d->k &&
-            in.part == part && in.ranks == ranks && in.pipe == pipe) {
+        if (!in.resident && same_key(in, pipe)) {
             *st = KAMI_OK;
             return &in;
         }
@@ -255,8 +264,8 @@ const BlockInstance* resolve_block(const kami_gemm_desc* d, kami_status* st) {
     // compiled instance of the requested partition
     if (ranks_defaulted) {
         for (const auto& in : registry())
-            if (in.prec == d->precision && in.m == d->m && in.n == d->n && in.k == d->k &&
-                in.part == part && in.pipe == pipe) {
+            if (!in.resident && in.prec == d->precision && in.m == d->m && in.n == d->n &&
+                in.k == d->k && in.part == part && in.pipe == pipe) {
                 *st = KAMI_OK;
                 return &in;
             }
@@ -264,8 +273,8 @@ const BlockInstance* resolve_block(const kami_gemm_desc* d, kami_status* st) {
     // AUTO falls back to any compiled per-problem instance of the shape
     if (d->partition == KAMI_PART_AUTO) {
         for (const auto& in : registry())
-            if (in.prec == d->precision && in.m == d->m && in.n == d->n && in.k == d->k &&
-                in.pipe == 1) {
+            if (!in.resident && in.prec == d->precision && in.m == d->m && in.n == d->n &&
+                in.k == d->k && in.pipe == 1) {
                 *st = KAMI_OK;
                 return &in;
             }
@@ -425,6 +434,7 @@ kami_status kami_block_gemm_autotune(const kami_gemm_desc* d, const void* A, con
     for (const auto& in : registry()) {
         if (in.prec != d->precision || in.m != d->m || in.n != d->n || in.k != d->k) continue;
         if (in.pipe == 2 && !pipe_allowed(d)) continue;
+        if (in.resident) continue;  // dispatched by iters under its streaming form's key
         kami_gemm_desc t = *d;
         t.partition = in.part;
         t.ranks = in.ranks;

@@ -255,3 +255,31 @@ def test_persistent_pipeline_matches_oracle(prec, s, r, dev):
                      alpha=2.0, beta=-1.0)
         ref = 2.0 * np.einsum("pik,pjk->pij", A, B) - C0
         np.testing.assert_array_equal(Cd.double().cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("prec,s,part,ranks", [
+    (K.PREC_F16_F32, 256, "1d", 8), (K.PREC_F16_F32, 256, "1d", 16), (K.PREC_BF16_F32, 256, "1d", 8),
+    (K.PREC_F64, 128, "1d", 8), (K.PREC_F16_F32, 192, "2d", 4), (K.PREC_BF16_F32, 192, "2d", 4),
+    (K.PREC_F16_F32, 192, "1d", 12)])
+@pytest.mark.parametrize("layout", ["nk", "kn"])
+def test_resident_forms_repeat_exactly(prec, s, part, ranks, layout, dev):
+    """iters > 1 dispatches the resident form of a key (all of K in one LDS stage; in 1D
+    A in registers); on integer inputs the result is exactly iters * alpha * A B^T + beta C."""
+    cfg = K.query(s, s, s, prec, partition=part, ranks=ranks, iters=3, b_layout=layout)
+    assert cfg["k_stage"] == s and cfg["waves_m"] == ranks
+    if (s, ranks) != (192, 12):  # 192^3 1D with 12 waves holds all of K in either form
+        assert K.query(s, s, s, prec, partition=part, ranks=ranks)["k_stage"] < s  # streaming
+    count = 6
+    rng = np.random.default_rng(s + ranks)
+    A = rng.integers(-2, 3, size=(count, s, s)).astype(np.float64)
+    Bnk = rng.integers(-2, 3, size=(count, s, s)).astype(np.float64)
+    C0 = rng.integers(-9, 10, size=(count, s, s)).astype(np.float64)
+    dt = {K.PREC_F16_F32: torch.float16, K.PREC_BF16_F32: torch.bfloat16, K.PREC_F64: torch.float64}[prec]
+    out_dt = torch.float64 if prec == K.PREC_F64 else torch.float32
+    Bin = Bnk if layout == "nk" else np.ascontiguousarray(Bnk.transpose(0, 2, 1))
+    Cd = torch.from_numpy(C0).to(dev).to(out_dt)
+    K.block_gemm(torch.from_numpy(A).to(dev).to(dt), torch.from_numpy(Bin).to(dev).to(dt), Cd,
+                 precision=prec, partition=part, ranks=ranks, b_layout=layout, alpha=2.0,
+                 beta=-1.0, iters=3)
+    ref = 6.0 * np.einsum("pik,pjk->pij", A, Bnk) - C0
+    np.testing.assert_array_equal(Cd.double().cpu().numpy(), ref)
