@@ -97,6 +97,26 @@ def measured_traffic(path, cfg, m, count):
     return rec["traffic_bytes"]
 
 
+def measured_lds(path, cfg, m, count):
+    """LDS traffic of the path's kernel from the SQ_INSTS_LDS record in profiles/lds.json
+    (same instance and workload only): bytes per FLOP where every LDS access of the
+    kernel has one width, else the instruction count alone; None without a record."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "lds.json")) as f:
+            rec = json.load(f).get({"native": "w3_native", "split": "w3_split"}.get(path, ""))
+    except (OSError, ValueError):
+        return None
+    if not rec or m != 128 or count != 4096:
+        return None
+    if any(cfg.get(k) != v for k, v in rec.get("config", {}).items()):
+        return None
+    out = {"instructions_per_launch": rec["SQ_INSTS_LDS"],
+           "bank_conflict_cycles": rec["SQ_LDS_BANK_CONFLICT"], "source": "rocprofv3 --pmc SQ_INSTS_LDS"}
+    if rec.get("bytes_per_instruction"):
+        out["bytes_per_flop"] = rec["SQ_INSTS_LDS"] * rec["bytes_per_instruction"] / rec["flop_per_launch"]
+    return out
+
+
 def rel_errors(C, A, B, sample):
     """Max errors of C against the compensated (Dot2) oracle on the first `sample`
     problems: componentwise |dC| / (|A||B|) and normwise max|dC| / max|C*| per problem."""
@@ -182,6 +202,7 @@ def main():
     r = results[chosen]
     ms = r["ms"]
     traffic = measured_traffic(chosen, r["config"], m, count)
+    lds = measured_lds(chosen, r["config"], m, count)
     if chosen == "native":
         roof = {"bound": "mfma", "achieved": r["tflops"], "peak": PEAK["f64_mfma_tflops"],
                 "unit": "TFLOP/s", "frac": r["tflops"] / PEAK["f64_mfma_tflops"],
@@ -209,6 +230,7 @@ def main():
                    "m": m, "n": n, "k": k, "problems": count,
                    "partition": r["config"]["partition"], "ranks": r["config"]["ranks"]},
         "roofline": roof,
+        "lds": lds,
         "cpu_baseline": cpu,
         "paths": {p: ({kk: vv for kk, vv in v.items() if kk != "config"} if v.get("available")
                       else v) for p, v in results.items()},
