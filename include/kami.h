@@ -225,7 +225,8 @@ kami_status kami_tune_clear(void);
  * B dense: KAMI_B_NK -> B stored n x K row-major (ldb >= K), C(i,j) = sum_k A(i,k) B[j*ldb+k],
  * the reference's layout; KAMI_B_KN -> B stored K x n (ldb >= n).  ldb / ldc of 0 mean
  * packed.  n must be a multiple of 16; bs 16 or 32.  precision: KAMI_PREC_F16_F32 /
- * KAMI_PREC_BF16_F32 (FP16/BF16 A and B, float C) or KAMI_PREC_F64.
+ * KAMI_PREC_BF16_F32 (FP16/BF16 A and B, float C) or KAMI_PREC_F64.  KAMI_ENOTSUP when
+ * mb * ceil(n / 64) reaches 2^24 (one workgroup per block row and 64 columns).
  * Replaces block_spmm_{1d,2d,3d}_half_mma (src/sparse/spmm/2d/block_spmm_2d_half_mma.cu:72)
  * and its host-side re-tiling bsr2tilebsr (:264-291). */
 kami_status kami_bsr_spmm(const kami_bsr* A, const void* B, int32_t n, int64_t ldb,

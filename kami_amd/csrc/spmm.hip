@@ -7,10 +7,18 @@
 // over workgroups:
 //   * one workgroup per (block row i, 64-column tile of C); wave w owns the 16 columns
 //     [16 w, 16 w + 16) of the tile and all bs rows of the block row;
-//   * for every stored block (i, kb) the workgroup stages the A block (bs x bs) and the
-//     B slice (64 columns x bs values of K, from B stored N x K or K x N) in LDS, rows
-//     padded to the conflict-free stride; the next block is loaded into registers while the current one
-//     is multiplied (double-buffered LDS, one barrier per block);
+//   * only the A block is shared by the four waves, so only it is staged in LDS (rows
+//     padded to the conflict-free stride, double-buffered, one barrier per block); each
+//     lane loads its own B values straight into MFMA fragments: lane (c, g) holds column c
+//     and the contiguous k range [g bs/4, (g+1) bs/4) of the block, and the A block is
+//     written to LDS with its columns permuted to match, so the products are those of the
+//     block's k values in another order;
+//   * the next block's A piece and B fragments are loaded while the current block is
+//     multiplied; the column indices of a row are read 64 at a time into a register and
+//     taken out with readlane, so no scalar load waits in the loop;
+//   * workgroups are numbered so that each XCD (workgroup L runs on XCD L mod 8) takes a
+//     contiguous run of (column tile, block row) pairs: its L2 sees the B columns of few
+//     tiles;
 //   * FP16 / BF16 inputs accumulate in FP32, FP64 in FP64; block rows without stored
 //     blocks still write beta * C.
 #include <algorithm>
@@ -24,16 +32,23 @@ struct SpmmGeom {
     using in_t = typename AT::in_t;
     static constexpr int NT = 256, TN = 64;  // C tile: BS rows x TN columns, 4 waves
     static constexpr int ELT = (int)sizeof(in_t), VE = 16 / ELT;
-    static constexpr int ROWB = lds_row_bytes(BS * ELT, AT::FRAG_BYTES);
-    static constexpr int A_BYTES = BS * ROWB;
-    static constexpr int STAGE = (BS + TN) * ROWB;
-    static constexpr int CPR = BS / VE;      // 16-byte pieces per staged row
-    static constexpr int NA = BS * CPR;      // pieces of the A block
-    static constexpr int NB = TN * CPR;      // pieces of the B slice (either layout)
-    static constexpr int NP = (NA + NB + NT - 1) / NT;
+    static constexpr int FB = AT::FRAG_BYTES;
+    static constexpr int ROWB = lds_row_bytes(BS * ELT, FB);
+    static constexpr int STAGE = BS * ROWB;  // one A block
+    static constexpr int CPR = BS / VE;      // 16-byte pieces per A row
+    static constexpr int NA = BS * CPR;      // pieces of an A block
+    static constexpr int NPA = (NA + NT - 1) / NT;
     static constexpr int IM = BS / 16;
+    static constexpr int NKS = BS / AT::KSTEP;  // MFMA k-steps per block
+    static constexpr int KPL = BS / 4;          // k values of B one lane holds per block
     static_assert(BS % AT::KSTEP == 0 && BS % 16 == 0, "block size");
+    static_assert(KPL == NKS * AT::KV, "four lane groups split the block's k range");
+    static_assert(FB == 16 || NKS == 1, "the k permutation moves 16-byte fragments");
     static_assert(TN == 16 * (NT / 64), "one 16-column atom per wave");
+    // LDS piece of A-row piece j: piece j holds k values [j KV, j KV + KV) = lane group
+    // g = j / NKS, k-step t = j % NKS, which the fragment read of step t, group g finds at
+    // piece 4 t + g.
+    __device__ static constexpr int a_piece(int j) { return (j % NKS) * 4 + j / NKS; }
 };
 
 struct SpmmArgs {
@@ -55,13 +70,22 @@ __global__ void __launch_bounds__(256) spmm_kernel(SpmmArgs a) {
     using out_t = typename AT::out_t;
     using frag_t = typename AT::frag_t;
     using accv_t = typename AT::accv_t;
-    constexpr int FB = AT::FRAG_BYTES;
+    constexpr int FB = AT::FRAG_BYTES, KV = AT::KV;
+    struct elems_t {
+        in_t v[KV];
+    };
     __shared__ __attribute__((aligned(16))) unsigned char smem[2 * G::STAGE];
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int i = blockIdx.x;                     // block row of A and C
-    const int n0 = blockIdx.y * G::TN;            // first column of the C tile
-    const bool wave_live = n0 + wave * 16 < a.n;  // wave-uniform
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, grp = lane >> 4;
+    // XCD x = L mod 8 takes the contiguous run [x q + min(x, r), ...) of the tile-major
+    // order, q = total / 8, r = total mod 8 (a bijection of [0, total)).
+    const uint32_t L = blockIdx.x, total = gridDim.x;
+    const uint32_t x = L % 8, q8 = total / 8, r8 = total % 8;
+    const uint32_t P = x * q8 + (x < r8 ? x : r8) + L / 8;
+    const int i = (int)(P % (uint32_t)a.mb);      // block row of A and C
+    const int n0 = (int)(P / (uint32_t)a.mb) * G::TN;  // first column of the C tile
+    const int col = n0 + wave * 16 + (lane & 15);
+    const bool wave_live = n0 + wave * 16 < a.n;  // wave-uniform; n is a multiple of 16
     const in_t* __restrict__ vals = reinterpret_cast<const in_t*>(a.vals);
     const in_t* __restrict__ B = reinterpret_cast<const in_t*>(a.B);
     // stored blocks of row i (clamped to the arrays the caller described)
@@ -69,49 +93,56 @@ __global__ void __launch_bounds__(256) spmm_kernel(SpmmArgs a) {
     const int64_t beg = rb < 0 ? 0 : rb;
     const int64_t end = re > a.nnzb ? a.nnzb : re;
 
-    u4_t r[G::NP];
-    auto load = [&](int64_t q) {
-        const int kb = a.colidx[q];
-        const bool kb_ok = kb >= 0 && kb < a.nb;
+    // column indices [cbase, cbase + 64) of the row, one per lane
+    int64_t cbase = beg;
+    int cvec = beg + lane < end ? a.colidx[beg + lane] : -1;
+    auto block_col = [&](int64_t q) {  // q uniform, called with q = beg, beg + 1, ...
+        if (q - cbase >= 64) {
+            cbase += 64;
+            cvec = cbase + lane < end ? a.colidx[cbase + lane] : -1;
+        }
+        return __builtin_amdgcn_readlane(cvec, (int)(q - cbase));
+    };
+
+    u4_t ra[G::NPA];
+    auto load_a = [&](int64_t q) {
 #pragma unroll
-        for (int p = 0; p < G::NP; ++p) {
+        for (int p = 0; p < G::NPA; ++p) {
             const int idx = tid + p * G::NT;
-            u4_t v = {0, 0, 0, 0};
-            if (idx < G::NA) {
-                v = load_raw<16>(vals + q * BS * BS + (idx / G::CPR) * BS + (idx % G::CPR) * G::VE);
-            } else if (idx < G::NA + G::NB && kb_ok) {
-                const int j = idx - G::NA;
-                if (!a.b_kn) {
-                    const int nn = j / G::CPR, c = j % G::CPR;
-                    if (n0 + nn < a.n)
-                        v = load_raw<16>(B + (int64_t)(n0 + nn) * a.ldb + (int64_t)kb * BS + c * G::VE);
-                } else {
-                    const int kr = j / (G::TN / G::VE), jc = (j % (G::TN / G::VE)) * G::VE;
-                    if (n0 + jc < a.n)
-                        v = load_raw<16>(B + ((int64_t)kb * BS + kr) * a.ldb + n0 + jc);
-                }
-            }
-            r[p] = v;
+            if (idx < G::NA)
+                ra[p] = load_raw<16>(vals + q * BS * BS + (idx / G::CPR) * BS + (idx % G::CPR) * G::VE);
         }
     };
-    auto write = [&](int buf) {
+    auto write_a = [&](int buf) {
         unsigned char* base = smem + buf * G::STAGE;
 #pragma unroll
-        for (int p = 0; p < G::NP; ++p) {
+        for (int p = 0; p < G::NPA; ++p) {
             const int idx = tid + p * G::NT;
-            if (idx < G::NA) {
-                store_raw<16>(base + (idx / G::CPR) * G::ROWB + (idx % G::CPR) * 16, r[p]);
-            } else if (idx < G::NA + G::NB) {
-                const int j = idx - G::NA;
-                if (!a.b_kn) {
-                    store_raw<16>(base + G::A_BYTES + (j / G::CPR) * G::ROWB + (j % G::CPR) * 16, r[p]);
-                } else {
-                    const int kr = j / (G::TN / G::VE), jc = (j % (G::TN / G::VE)) * G::VE;
-                    const in_t* e = reinterpret_cast<const in_t*>(&r[p]);
+            if (idx < G::NA)
+                store_raw<16>(base + (idx / G::CPR) * G::ROWB + G::a_piece(idx % G::CPR) * 16, ra[p]);
+        }
+    };
+    // B fragments of block column kb for this lane: step t holds k = kb bs + g KPL + t KV + v
+    auto load_b = [&](int kb, frag_t (&f)[G::NKS]) {
+        elems_t z;
 #pragma unroll
-                    for (int q = 0; q < G::VE; ++q)
-                        *reinterpret_cast<in_t*>(base + G::A_BYTES + (jc + q) * G::ROWB + kr * G::ELT) = e[q];
-                }
+        for (int v = 0; v < KV; ++v) z.v[v] = in_t(0);
+#pragma unroll
+        for (int t = 0; t < G::NKS; ++t) f[t] = bitcast<frag_t>(z);
+        if (!wave_live || kb < 0 || kb >= a.nb) return;  // wave-uniform
+        const int64_t k0 = (int64_t)kb * BS + grp * G::KPL;
+        if (!a.b_kn) {
+            const in_t* p = B + (int64_t)col * a.ldb + k0;
+#pragma unroll
+            for (int t = 0; t < G::NKS; ++t) f[t] = bitcast<frag_t>(load_raw<FB>(p + t * KV));
+        } else {
+            const in_t* p = B + k0 * a.ldb + col;
+#pragma unroll
+            for (int t = 0; t < G::NKS; ++t) {
+                elems_t e;
+#pragma unroll
+                for (int v = 0; v < KV; ++v) e.v[v] = p[(int64_t)(t * KV + v) * a.ldb];
+                f[t] = bitcast<frag_t>(e);
             }
         }
     };
@@ -123,39 +154,44 @@ __global__ void __launch_bounds__(256) spmm_kernel(SpmmArgs a) {
         for (int v = 0; v < 4; ++v) acc[im][v] = 0;
 
     if (beg < end) {
-        load(beg);
-        write(0);
+        frag_t bc[G::NKS], bn[G::NKS];
+        load_a(beg);
+        load_b(block_col(beg), bc);
+        write_a(0);
         __syncthreads();
         int buf = 0;
         for (int64_t q = beg; q < end; ++q) {
             const bool next = q + 1 < end;
-            if (next) load(q + 1);
+            if (next) {
+                load_a(q + 1);
+                load_b(block_col(q + 1), bn);
+            }
             if (wave_live) {
                 const unsigned char* base = smem + buf * G::STAGE;
 #pragma unroll
-                for (int ks = 0; ks < BS / AT::KSTEP; ++ks) {
-                    const int kk = (ks * AT::KSTEP + AT::KV * (lane >> 4)) * G::ELT;
-                    const frag_t bf = bitcast<frag_t>(
-                        load_raw<FB>(base + G::A_BYTES + (wave * 16 + (lane & 15)) * G::ROWB + kk));
+                for (int ks = 0; ks < G::NKS; ++ks) {
+                    const int kk = (ks * AT::KSTEP + KV * grp) * G::ELT;
+                    frag_t af[G::IM];
 #pragma unroll
-                    for (int im = 0; im < G::IM; ++im) {
-                        const frag_t af = bitcast<frag_t>(
-                            load_raw<FB>(base + (im * 16 + (lane & 15)) * G::ROWB + kk));
+                    for (int im = 0; im < G::IM; ++im)
+                        af[im] = bitcast<frag_t>(load_raw<FB>(base + (im * 16 + (lane & 15)) * G::ROWB + kk));
 #pragma unroll
-                        for (int e = 0; e < AT::NSUB; ++e) AT::mma_sub(acc[im], af, bf, e);
-                    }
+                    for (int e = 0; e < AT::NSUB; ++e)
+#pragma unroll
+                        for (int im = 0; im < G::IM; ++im) AT::mma_sub(acc[im], af[im], bc[ks], e);
                 }
             }
-            if (next) write(buf ^ 1);
+            if (next) write_a(buf ^ 1);
             __syncthreads();
             buf ^= 1;
+#pragma unroll
+            for (int t = 0; t < G::NKS; ++t) bc[t] = bn[t];
         }
     }
 
     if (!wave_live) return;
     out_t* __restrict__ C = reinterpret_cast<out_t*>(a.C);
     const acc_t alpha = (acc_t)a.alpha, beta = (acc_t)a.beta;
-    const int col = n0 + wave * 16 + (lane & 15);
 #pragma unroll
     for (int im = 0; im < G::IM; ++im)
 #pragma unroll
@@ -199,7 +235,8 @@ kami_status kami_bsr_spmm(const kami_bsr* A, const void* B, int32_t n, int64_t l
         return KAMI_EINVAL;
     if ((ldb * ei) % 16 || ((uintptr_t)B % 16) || ((uintptr_t)C % eo)) return KAMI_EINVAL;
     if (A->nnzb > 0 && ((uintptr_t)A->vals % 16)) return KAMI_EINVAL;
-    if ((n + 63) / 64 > 65535) return KAMI_ENOTSUP;
+    const int64_t wgs = (int64_t)A->mb * ((n + 63) / 64);
+    if (wgs >= ((int64_t)1 << 24)) return KAMI_ENOTSUP;  // work-items of one dispatch < 2^32
     kami_status st = device_check();
     if (st != KAMI_OK) return st;
     SpmmArgs a;
@@ -217,7 +254,7 @@ kami_status kami_bsr_spmm(const kami_bsr* A, const void* B, int32_t n, int64_t l
     a.C = C;
     a.alpha = alpha;
     a.beta = beta;
-    const dim3 grid((unsigned)A->mb, (unsigned)((n + 63) / 64));
+    const dim3 grid((unsigned)wgs);
     hipStream_t s = (hipStream_t)stream;
 #define KAMI_SPMM(BS, AT) hipLaunchKernelGGL((spmm_kernel<BS, AT>), grid, dim3(256), 0, s, a)
     if (precision == KAMI_PREC_F16_F32) {
