@@ -29,34 +29,45 @@
 
 namespace kami {
 
-// Largest |digit| sum bound of level g for BETA-bit signed digits over KC terms, in
-// units of 2^0, compared against 2^24 (the FP32 integer range).
-constexpr double level_bound(int beta, int S, int g, int kc) {
+// Largest |digit| sum bound of level g over KC terms, in units of 2^0, compared against
+// 2^24 (the FP32 integer range).  The first digit has b1 bits (|d_1| <= 2^b1), the later
+// ones BETA bits with round to nearest (|d_s| <= 2^(BETA-1)).
+constexpr double level_bound(int beta, int S, int g, int kc, int b1) {
     double tot = 0;
     for (int s = 1; s <= S; ++s) {
         const int t = g - s;
         if (t < 1 || t > S) continue;
-        const double bs = (s == 1) ? (double)(1 << beta) : (double)(1 << (beta - 1));
-        const double bt = (t == 1) ? (double)(1 << beta) : (double)(1 << (beta - 1));
+        const double bs = (s == 1) ? (double)(1 << b1) : (double)(1 << (beta - 1));
+        const double bt = (t == 1) ? (double)(1 << b1) : (double)(1 << (beta - 1));
         tot += kc * bs * bt;
     }
     return tot;
 }
+constexpr double level_bound(int beta, int S, int g, int kc) { return level_bound(beta, S, g, kc, beta); }
 
 // Every level whose FP32 rounding could matter is exact: a level that may exceed 2^24
-// is allowed only when its worst rounding, 2^-24 * bound * 2^(-beta g) in units of
-// 2^(e_i + f_j), stays below 2^-50.
-constexpr bool levels_exact(int beta, int S, int G, int kc) {
-    for (int g = 2; g <= G; ++g) {
-        const double b = level_bound(beta, S, g, kc);
+// is allowed only when its worst rounding, 2^-24 * bound * 2^(-beta g + 2 (beta - b1)) in
+// units of 2^(e_i + f_j), stays below 2^-50.
+constexpr bool levels_exact_range(int beta, int S, int g0, int G, int kc, int b1) {
+    for (int g = g0; g <= G; ++g) {
+        const double b = level_bound(beta, S, g, kc, b1);
         if (b <= 16777216.0) continue;
         double rel = b;
-        for (int q = 0; q < 24 + beta * g; ++q) rel *= 0.5;
+        for (int q = 0; q < 24 + beta * g - 2 * (beta - b1); ++q) rel *= 0.5;
         double lim = 1.0;
         for (int q = 0; q < 50; ++q) lim *= 0.5;
         if (rel > lim) return false;
     }
     return true;
+}
+constexpr bool levels_exact(int beta, int S, int G, int kc) {
+    return levels_exact_range(beta, S, 2, G, kc, beta);
+}
+
+// One level's part of levels_exact: its sum over kc terms is exact in FP32, or its worst
+// rounding is below 2^-50 in units of 2^(e_i + f_j).
+constexpr bool levels_exact_one(int beta, int S, int g, int kc, int b1) {
+    return levels_exact_range(beta, S, g, g, kc, b1);
 }
 
 // 2^e as a compile-time constant (exact for the exponents used here).
@@ -101,6 +112,7 @@ struct SplitArgs {
     // persistent form only: K chunks per problem and the grid of BM x BN tiles of C per
     // problem (1 x 1 for a block problem)
     int32_t nch, tiles_m, tiles_n;
+    int32_t iters;  // resident form: repetitions of the product
 };
 
 __device__ inline int exp_of(double x) {
@@ -497,6 +509,213 @@ split_pipe_kernel(SplitArgs args, int64_t count) {
     }
 }
 
+// Resident-repeat geometry: the whole problem (BM x BK of A, BN x BK of B) is sliced
+// once into LDS, S digit planes of full-K rows.  The first digit has B1 <= BETA bits
+// (x 2^(B1 - e) = sum_s d_s 2^(-BETA (s-1))): one bit less than BETA keeps more level
+// sums exact over the whole K, at one bit of the representation.
+template <class SAT, int BM_, int BN_, int BK_, int PM_, int PN_, int BETA_, int S_, int G_,
+          int B1_>
+struct SplitResGeom {
+    static constexpr int BM = BM_, BN = BN_, BK = BK_, PM = PM_, PN = PN_;
+    static constexpr int BETA = BETA_, S = S_, G = G_, B1 = B1_;
+    static constexpr int WAVES = PM * PN, NT = WAVES * kWave;
+    static constexpr int TM = BM / PM, TN = BN / PN, IM = TM / 16, JN = TN / 16;
+    static constexpr int KC = 32, NCH = BK / KC;
+    static constexpr int ROWB = lds_row_bytes(BK * 2, 16);  // BK digits per row
+    static constexpr int PLANE = (BM + BN) * ROWB;          // rows of A, then rows of B
+    static constexpr int EXP_OFF = S * PLANE;
+    static constexpr int LDS = EXP_OFF + (BM + BN) * 4;
+    static constexpr int CPR = BK / 2;  // 16-byte pieces (two doubles) per row
+    static constexpr int PA = BM * CPR / NT, PB = BN * CPR / NT;
+    static_assert(BM % PM == 0 && BN % PN == 0 && TM % 16 == 0 && TN % 16 == 0, "tiling");
+    static_assert(BK % KC == 0 && CPR <= 64 && 64 % CPR == 0, "a row's pieces sit in one wave");
+    static_assert((BM * CPR) % NT == 0 && (BN * CPR) % NT == 0, "piece distribution");
+    static_assert(LDS <= 160 * 1024, "LDS budget");
+    static_assert(B1 <= BETA && levels_exact_range(BETA, S, 2, G, KC, B1),
+                  "slice digits would not sum exactly in FP32");
+    // weight of level g in units of 2^(e_i + f_j)
+    static constexpr double weight(int g) { return pow2(-BETA * g + 2 * (BETA - B1)); }
+    // Level g may run its FP32 sum over the whole K (drained once per repetition) when
+    // that sum is exact or its worst rounding is below 2^-50 of 2^(e_i + f_j); the other
+    // levels are drained after every 32-wide chunk.
+    static constexpr bool whole_k(int g) { return levels_exact_one(BETA, S, g, BK, B1); }
+    // The two smallest levels are combined in FP32 (T_(G-1) + T_G 2^-BETA) before their
+    // one conversion; the rounding of that sum must stay below 2^-50 as well.
+    static_assert(G >= 3 && whole_k(G) && whole_k(G - 1) &&
+                      level_bound(BETA, S, G - 1, BK, B1) * pow2(-24) * weight(G - 1) <= pow2(-50),
+                  "the two smallest levels are combined in FP32");
+};
+
+// Maximum exponent over the CPR consecutive lanes that hold one row (CPR = 16 or 32 here;
+// any power of two up to 64), the all-zero sentinel mapped to 0 only after the full
+// reduction.
+template <int CPR>
+__device__ inline int row_max_exp(int e) {
+#pragma unroll
+    for (int off = 1; off < CPR; off <<= 1) e = max(e, __shfl_xor(e, off, 64));
+    return e == -4096 ? 0 : e;
+}
+
+// Resident-repeat form (`iters` > 1, the reference's NUM_ITER loop): one workgroup per
+// problem.  The operands are read once, the exponents taken per row over the full K, and
+// every digit plane of A and B written to LDS once; each wave then loads the digits of its
+// rows of A and of B into registers, and the `iters` repetitions run only the level
+// products and their FP64 recombination.  Level sums T_g that stay exact over the whole
+// K (whole_k) are accumulated by the MFMAs across the chunks and added into the FP64
+// accumulator once per repetition, the others after every 32-wide chunk; the two smallest
+// levels are combined in FP32 first.  Each addition is acc += T_g weight(g).
+template <class SAT, class G>
+__global__ void __launch_bounds__(G::NT)
+split_resident_kernel(SplitArgs args) {
+    using frag_t = typename SAT::frag_t;
+    using accv_t = typename SAT::accv_t;
+    __shared__ __attribute__((aligned(16))) unsigned char smem[G::LDS];
+    int* const expo = reinterpret_cast<int*>(smem + G::EXP_OFF);  // [BM] then [BN]
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int pn = wave % G::PN, pm = wave / G::PN;
+    const int64_t prob = blockIdx.x;
+    const double* __restrict__ A = args.A + prob * args.sa;
+    const double* __restrict__ B = args.B + prob * args.sb;
+    double* __restrict__ C = args.C + prob * args.sc;
+    const int64_t lda = args.lda, ldb = args.ldb, ldc = args.ldc;
+
+    // ---- operands: piece q = tid + i NT is row q / CPR, k pair q % CPR ----
+    u4_t xa[G::PA], xb[G::PB];
+#pragma unroll
+    for (int i = 0; i < G::PA; ++i) {
+        const int q = tid + i * G::NT;
+        xa[i] = load_raw<16>(A + (q / G::CPR) * lda + 2 * (q % G::CPR));
+    }
+#pragma unroll
+    for (int i = 0; i < G::PB; ++i) {
+        const int q = tid + i * G::NT;
+        xb[i] = load_raw<16>(B + (q / G::CPR) * ldb + 2 * (q % G::CPR));
+    }
+    auto slice_piece = [&](const u4_t& raw, int row, int kp, unsigned char* plane0) {
+        const d2_t x = bitcast<d2_t>(raw);
+        const int e = row_max_exp<G::CPR>(max(exp_of(x[0]), exp_of(x[1])));
+        if (kp == 0) reinterpret_cast<int*>(smem + G::EXP_OFF)[row] = e;
+        double r0 = __builtin_amdgcn_ldexp(x[0], G::B1 - e);
+        double r1 = __builtin_amdgcn_ldexp(x[1], G::B1 - e);
+#pragma unroll
+        for (int s = 0; s < G::S; ++s) {
+            const double q0 = __builtin_rint(r0), q1 = __builtin_rint(r1);
+            *reinterpret_cast<uint32_t*>(plane0 + s * G::PLANE + row * G::ROWB + kp * 4) =
+                pack2<SAT>(q0, q1);
+            if (s + 1 < G::S) {
+                r0 = (r0 - q0) * (double)(1 << G::BETA);
+                r1 = (r1 - q1) * (double)(1 << G::BETA);
+            }
+        }
+    };
+#pragma unroll
+    for (int i = 0; i < G::PA; ++i) {
+        const int q = tid + i * G::NT;
+        slice_piece(xa[i], q / G::CPR, q % G::CPR, smem);
+    }
+#pragma unroll
+    for (int i = 0; i < G::PB; ++i) {
+        const int q = tid + i * G::NT;
+        slice_piece(xb[i], G::BM + q / G::CPR, q % G::CPR, smem);
+    }
+    __syncthreads();
+
+    // digits of the wave's rows of A and B, every chunk and slice, held in registers
+    frag_t af[G::NCH][G::S][G::IM], bfr[G::NCH][G::S][G::JN];
+#pragma unroll
+    for (int c = 0; c < G::NCH; ++c)
+#pragma unroll
+        for (int s = 0; s < G::S; ++s) {
+#pragma unroll
+            for (int im = 0; im < G::IM; ++im)
+                af[c][s][im] = bitcast<frag_t>(load_raw<16>(
+                    smem + s * G::PLANE + (pm * G::TM + im * 16 + (lane & 15)) * G::ROWB +
+                    c * G::KC * 2 + 16 * (lane >> 4)));
+#pragma unroll
+            for (int jn = 0; jn < G::JN; ++jn)
+                bfr[c][s][jn] = bitcast<frag_t>(load_raw<16>(
+                    smem + s * G::PLANE + (G::BM + pn * G::TN + jn * 16 + (lane & 15)) * G::ROWB +
+                    c * G::KC * 2 + 16 * (lane >> 4)));
+        }
+
+    double acc[G::IM][G::JN][4];
+#pragma unroll
+    for (int i = 0; i < G::IM; ++i)
+#pragma unroll
+        for (int j = 0; j < G::JN; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[i][j][v] = 0.0;
+
+    for (int it = 0; it < args.iters; ++it) {
+#pragma unroll
+        for (int jn = 0; jn < G::JN; ++jn) {
+            accv_t T[G::IM][G::G - 1];  // level g of atom row im in T[im][g - 2]
+#pragma unroll
+            for (int im = 0; im < G::IM; ++im)
+#pragma unroll
+                for (int l = 0; l < G::G - 1; ++l) T[im][l] = accv_t{0, 0, 0, 0};
+#pragma unroll
+            for (int c = 0; c < G::NCH; ++c) {
+#pragma unroll
+                for (int t = 1; t <= G::S; ++t) {
+                    if (t + 1 > G::G) continue;
+#pragma unroll
+                    for (int s = 1; s <= G::S; ++s)
+                        if (s + t <= G::G)
+#pragma unroll
+                            for (int im = 0; im < G::IM; ++im)
+                                SAT::mma(T[im][s + t - 2], af[c][s - 1][im], bfr[c][t - 1][jn]);
+                }
+                // levels that are exact only within a chunk
+#pragma unroll
+                for (int g = G::G; g >= 2; --g) {
+                    if (G::whole_k(g)) continue;
+#pragma unroll
+                    for (int im = 0; im < G::IM; ++im)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            acc[im][jn][v] = __builtin_fma((double)T[im][g - 2][v],
+                                                           G::weight(g), acc[im][jn][v]);
+                            T[im][g - 2][v] = 0.0f;
+                        }
+                }
+            }
+            // whole-K levels, the two smallest combined in FP32 first
+#pragma unroll
+            for (int im = 0; im < G::IM; ++im)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const float low = __builtin_fmaf(T[im][G::G - 2][v], (float)pow2(-G::BETA),
+                                                     T[im][G::G - 3][v]);
+                    double a = __builtin_fma((double)low, G::weight(G::G - 1), acc[im][jn][v]);
+#pragma unroll
+                    for (int g = G::G - 2; g >= 2; --g)
+                        if (G::whole_k(g)) a = __builtin_fma((double)T[im][g - 2][v], G::weight(g), a);
+                    acc[im][jn][v] = a;
+                }
+        }
+    }
+
+    // ---- rescale and store ----
+#pragma unroll
+    for (int im = 0; im < G::IM; ++im)
+#pragma unroll
+        for (int jn = 0; jn < G::JN; ++jn) {
+            const int col = pn * G::TN + jn * 16 + (lane & 15);
+            const int ecol = expo[G::BM + col];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = pm * G::TM + im * 16 + 4 * (lane >> 4) + v;
+                double val = __builtin_amdgcn_ldexp(acc[im][jn][v], expo[row] + ecol);
+                double* dst = C + row * ldc + col;
+                val *= args.alpha;
+                if (args.beta != 0.0) val += args.beta * (*dst);
+                *dst = val;
+            }
+        }
+}
+
 template <class SAT, class G>
 hipError_t launch_split(const SplitArgs& a, int64_t count, hipStream_t s) {
     hipLaunchKernelGGL((split_gemm_kernel<SAT, G>), dim3((unsigned)count), dim3(G::NT), 0, s, a);
@@ -517,21 +736,37 @@ hipError_t launch_split_pipe(const SplitArgs& a, int64_t count, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <class SAT, class G>
+hipError_t launch_split_resident(const SplitArgs& a, int64_t count, hipStream_t s) {
+    hipLaunchKernelGGL((split_resident_kernel<SAT, G>), dim3((unsigned)count), dim3(G::NT), 0, s,
+                       a);
+    return hipGetLastError();
+}
+
 typedef hipError_t (*split_launch_fn)(const SplitArgs&, int64_t, hipStream_t);
 
 struct SplitInstance {
     int prec, m, n, k, slices, pm, pn, threads, lds;
     split_launch_fn launch;
-    int pipe;  // 1: one workgroup per problem, exponents over full K; 2: persistent
+    int pipe;      // 1: one workgroup per problem, exponents over full K; 2: persistent
+    int resident;  // 1: resident-repeat form, used for iters > 1 only
 };
 
 template <class SAT, int PREC, int BM, int BN, int BK, int PM, int PN, int BETA, int S, int G>
 static void add_split(std::vector<SplitInstance>& v) {
     using Geo = SplitGeom<SAT, BM, BN, BK, PM, PN, BETA, S, G>;
     v.push_back(SplitInstance{PREC, BM, BN, BK, S, PM, PN, Geo::NT, Geo::LDS,
-                              &launch_split<SAT, Geo>, 1});
+                              &launch_split<SAT, Geo>, 1, 0});
     v.push_back(SplitInstance{PREC, BM, BN, BK, S, PM, PN, Geo::NT, Geo::LDS,
-                              &launch_split_pipe<SAT, Geo>, 2});
+                              &launch_split_pipe<SAT, Geo>, 2, 0});
+}
+
+template <class SAT, int PREC, int BM, int BN, int BK, int PM, int PN, int BETA, int S, int G,
+          int B1>
+static void add_split_resident(std::vector<SplitInstance>& v) {
+    using Geo = SplitResGeom<SAT, BM, BN, BK, PM, PN, BETA, S, G, B1>;
+    v.push_back(SplitInstance{PREC, BM, BN, BK, S, PM, PN, Geo::NT, Geo::LDS,
+                              &launch_split_resident<SAT, Geo>, 1, 1});
 }
 
 static const std::vector<SplitInstance>& split_registry() {
@@ -545,25 +780,30 @@ static const std::vector<SplitInstance>& split_registry() {
         // BF16 digits: 8 bits; default 7 slices (levels <= 8, 28 products)
         add_split<AtomBF16x32, BF, 128, 128, 128, 4, 2, 8, 7, 8>(r);
         add_split<AtomBF16x32, BF, 64, 64, 64, 2, 2, 8, 7, 8>(r);
+        // resident-repeat forms: the whole 64^3 problem's digit planes in LDS, 8 waves;
+        // FP16 with an 8-bit first digit, so that every level sum runs over all of K
+        add_split_resident<AtomF16x32, F16, 64, 64, 64, 4, 2, 9, 6, 7, 8>(r);
+        add_split_resident<AtomBF16x32, BF, 64, 64, 64, 4, 2, 8, 7, 8, 8>(r);
         return r;
     }();
     return v;
 }
 
-// pipeline 0 selects the persistent form.
+// pipeline 0 selects the persistent form for iters = 1 and the resident form (one
+// workgroup per problem) for iters > 1.
 static const SplitInstance* resolve_split(const kami_gemm_desc* d) {
     const int want = d->slices ? d->slices : (d->precision == KAMI_PREC_F64_SPLIT_F16 ? 6 : 7);
-    const int pipe = d->pipeline ? d->pipeline : 2;
+    const int rep = d->iters > 1 ? 1 : 0;
+    const int pipe = d->pipeline ? d->pipeline : (rep ? 1 : 2);
     for (const auto& in : split_registry())
         if (in.prec == d->precision && in.m == d->m && in.n == d->n && in.k == d->k &&
-            in.slices == want && in.pipe == pipe)
+            in.slices == want && in.pipe == pipe && in.resident == rep)
             return &in;
     return nullptr;
 }
 
 kami_status split_query(const kami_gemm_desc* d, kami_gemm_config* out) {
-    if (d->b_layout != KAMI_B_NK || d->partition == KAMI_PART_3D || d->iters != 1)
-        return KAMI_ENOTSUP;
+    if (d->b_layout != KAMI_B_NK || d->partition == KAMI_PART_3D) return KAMI_ENOTSUP;
     const SplitInstance* in = resolve_split(d);
     if (!in) return KAMI_ENOTSUP;
     out->partition = KAMI_PART_2D;
@@ -572,12 +812,12 @@ kami_status split_query(const kami_gemm_desc* d, kami_gemm_config* out) {
     out->waves_n = in->pn;
     out->waves_k = 1;
     out->threads = in->threads;
-    out->k_stage = 32;
+    out->k_stage = in->resident ? in->k : 32;
     out->lds_bytes = in->lds;
     out->slices = in->slices;
     out->instance = (int32_t)(in - split_registry().data());
     out->pipeline = in->pipe;
-    out->prefetch = 1;
+    out->prefetch = in->resident ? 0 : 1;
     return KAMI_OK;
 }
 
@@ -602,6 +842,7 @@ kami_status split_launch(const kami_gemm_desc* d, const BlockArgs& b, int64_t co
     a.nch = d->k / 32;
     a.tiles_m = 1;
     a.tiles_n = 1;
+    a.iters = b.iters;
     hipError_t e = in->launch(a, count, stream);
     if (e != hipSuccess) return hip_fail(e);
     return KAMI_OK;
@@ -640,6 +881,7 @@ kami_status split_gemm_tiled(const kami_gemm_desc* d, const BlockArgs& b, int64_
     a.nch = d->k / 32;
     a.tiles_m = d->m / tile;
     a.tiles_n = d->n / tile;
+    a.iters = 1;
     kami_status st = device_check();
     if (st != KAMI_OK) return st;
     hipError_t e = in->launch(a, count * tiles, stream);

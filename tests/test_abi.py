@@ -108,6 +108,16 @@ def test_split_precision_query_and_enotsup():
         with pytest.raises(K.KamiError) as e:
             K.query(128, 128, 128, K.PREC_F64_SPLIT_F16, **kw)
         assert e.value.status == _abi.ENOTSUP
+    # 64^3 with iters > 1 resolves to the resident-repeat form (whole K on chip); the
+    # persistent form takes iters = 1 only
+    for prec, slices in [(K.PREC_F64_SPLIT_F16, 6), (K.PREC_F64_SPLIT_BF16, 7)]:
+        cfg = K.query(64, 64, 64, prec, iters=3)
+        assert (cfg["k_stage"], cfg["pipeline"], cfg["prefetch"], cfg["slices"]) == (64, 1, 0, slices)
+        assert cfg["threads"] == 512 and cfg["lds_bytes"] <= 160 * 1024
+        assert K.query(64, 64, 64, prec, iters=1)["k_stage"] == 32
+        with pytest.raises(K.KamiError) as e:
+            K.query(64, 64, 64, prec, iters=3, pipeline=2)
+        assert e.value.status == _abi.ENOTSUP
 
 
 def test_batched_plan_validation_without_device():

@@ -149,6 +149,66 @@ def test_persistent_walks_many_problems(prec, slices, dev):
         np.testing.assert_array_equal(C, oracle.gemm_f64(A, B))
 
 
+# resident-repeat forms (iters > 1): 64^3, the default slice count of each digit type
+RESIDENT = [(K.PREC_F64_SPLIT_F16, 6), (K.PREC_F64_SPLIT_BF16, 7)]
+RIDS = ["f16-s6", "bf16-s7"]
+
+
+@pytest.mark.parametrize("prec,slices", RESIDENT, ids=RIDS)
+@pytest.mark.parametrize("iters", [2, 5])
+def test_resident_repeat_kat_exact(prec, slices, iters, dev):
+    """The resident form repeats the slice products `iters` times into one accumulator, as
+    the reference's NUM_ITER loop does; on integer inputs every digit, level sum and FP64
+    addition is exact, so C = iters * A B exactly (ramp and rand_r % 3 inputs)."""
+    cfg = K.query(64, 64, 64, prec, slices=slices, iters=iters)
+    assert cfg["k_stage"] == 64 and cfg["pipeline"] == 1 and cfg["prefetch"] == 0
+    s, count = 64, 3
+    ramp = np.tile(oracle.ramp(s * s).reshape(1, s, s), (count, 1, 1))
+    randr = oracle.randr_mod3(count * s * s).reshape(count, s, s) - 1.0
+    for A, B in [(ramp, ramp), (randr, ramp), (randr, randr * 7.0)]:
+        C = run(A, B, prec, slices, dev, iters=iters)
+        np.testing.assert_array_equal(C, iters * oracle.gemm_f64(A, B))
+
+
+@pytest.mark.parametrize("prec,slices", RESIDENT, ids=RIDS)
+def test_resident_repeat_log_uniform_normwise_1e13(prec, slices, dev):
+    rng = np.random.default_rng(29)
+    s, count, iters = 64, 64, 3
+    A = log_uniform(rng, (count, s, s))
+    B = log_uniform(rng, (count, s, s))
+    C = run(A, B, prec, slices, dev, iters=iters)
+    exact, _ = oracle.gemm_dot2(A, B)
+    assert normwise(C / iters, exact) <= 1e-13
+    # the one-workgroup-per-problem form (iters = 1) agrees far inside the bound; the
+    # resident FP16 form slices with an 8-bit first digit, so the two differ by a few ulps
+    C1 = run(A, B, prec, slices, dev, pipe=1)
+    assert normwise(C / iters, C1) <= 1e-14
+
+
+@pytest.mark.parametrize("prec,slices", RESIDENT, ids=RIDS)
+def test_resident_repeat_alpha_beta_and_adversarial_rows(prec, slices, dev):
+    rng = np.random.default_rng(31)
+    s, iters = 64, 2
+    A = rng.integers(-4, 5, (2, s, s)).astype(np.float64)
+    B = rng.integers(-4, 5, (2, s, s)).astype(np.float64)
+    C0 = rng.integers(-9, 10, (2, s, s)).astype(np.float64)
+    C = run(A, B, prec, slices, dev, C=C0, iters=iters, alpha=-0.5, beta=2.0)
+    np.testing.assert_array_equal(C, -0.5 * iters * oracle.gemm_f64(A, B) + 2.0 * C0)
+    A = rng.uniform(-1, 1, (2, s, s))
+    B = rng.uniform(-1, 1, (2, s, s))
+    A[0, 0, :] = 0.0
+    A[0, 2, :] = 1e-6 * rng.uniform(-1, 1, s)
+    A[0, 2, 5] = 1e6
+    A[1, 5, :] = 2.0 ** -40 * rng.uniform(-1, 1, s)
+    A[1, 5, 0] = 1.0
+    B[1] *= 1e200
+    A[1] *= 1e-200
+    C = run(A, B, prec, slices, dev, iters=iters)
+    exact, _ = oracle.gemm_dot2(A, B)
+    assert normwise(C / iters, exact) <= 1e-13
+    assert np.all(C[0, 0] == 0.0)
+
+
 def test_split_unsupported_requests_are_enotsup():
     for kw in [dict(b_layout="kn"), dict(partition="3d"), dict(iters=2), dict(slices=3)]:
         with pytest.raises(K.KamiError) as e:
