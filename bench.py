@@ -139,7 +139,7 @@ def main():
     ap.add_argument("--err-sample", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true",
-                    help="only the headline workload (skip the other four)")
+                    help="only the headline workload (skip the other workloads)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only; --gpus {args.gpus} is not supported "
@@ -240,6 +240,7 @@ def main():
         from kami_amd import workloads as W
         others = {}
         for name, fn in [("w1_f64_64cubed_x1024", W.run_w1), ("w2_f16_block_gemm", W.run_w2),
+                         ("w3r_resident_f64_64cubed_x4096", W.run_w3r),
                          ("w4_batched_uneven_x20000", W.run_w4), ("w5_block_spgemm", W.run_w5),
                          ("w6_block_spmm", W.run_w6)]:
             t0 = time.perf_counter()

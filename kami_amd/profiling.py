@@ -3,7 +3,7 @@
     python -m kami_amd.profiling copy --mib 1024         # plain device copy (calibration)
     python -m kami_amd.profiling w3 --path native        # FP64 128^3 x 4096 native
     python -m kami_amd.profiling w3 --path split         # same through FP16 slices
-    python -m kami_amd.profiling w1 | w4 --prec f16 | w5 | w6
+    python -m kami_amd.profiling w1 | w3r | w4 --prec f16 | w5 | w6
 
 Each run launches the kernel `--reps` times after one warm-up, so per-dispatch counters
 (FETCH_SIZE, WRITE_SIZE in KiB) can be divided by the algorithmic bytes of one launch.
@@ -20,7 +20,7 @@ import numpy as np
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["copy", "w1", "w3", "w4", "w5", "w6"])
+    ap.add_argument("what", choices=["copy", "w1", "w3", "w3r", "w4", "w5", "w6"])
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--path", choices=["native", "split"], default="native")
     ap.add_argument("--prec", choices=["f16", "f64"], default="f16")
@@ -58,6 +58,8 @@ def main(argv=None):
         return
     if args.what == "w1":
         W.run_w1(steps=args.reps, warmup=1)
+    elif args.what == "w3r":
+        W.run_w3r(steps=args.reps, warmup=1)
     elif args.what == "w4":
         W.run_w4(steps=args.reps, warmup=1, precisions=(args.prec,))
     elif args.what == "w5":

@@ -116,6 +116,37 @@ def run_w2(count=8192, steps=10, warmup=2, resident_tflop=4.0):
     return out
 
 
+# ------------------------------------------------------------------ w3r
+
+def run_w3r(count=4096, iters=200, steps=20, warmup=10):
+    """Resident-repeat FP64 block GEMM at 64^3 (the reference's NUM_ITER mode): operands
+    read once, `iters` products accumulated inside the kernel.  Native FP64 MFMA against
+    FP64 through FP16 / BF16 slices (digits made once; the slice products and their FP64
+    recombination repeated), log-uniform signed inputs as in the split workload."""
+    torch = _torch()
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(SEED + 5)
+    s = 64
+    A = torch.from_numpy(log_uniform_signed(rng, (count, s, s))).to(dev)
+    B = torch.from_numpy(log_uniform_signed(rng, (count, s, s))).to(dev)
+    C = torch.empty((count, s, s), dtype=torch.float64, device=dev)
+    ref = torch.bmm(A[:32], B[:32].transpose(1, 2).contiguous())
+    flops = 2.0 * s ** 3 * count * iters
+    out = {"workload": f"FP64 block GEMM 64x64x64 x{count}, log-uniform |x| in [1e-4, 1e4], "
+                       f"resident-repeat iters={iters}"}
+    for name, prec in [("native", K.PREC_F64), ("split_f16", K.PREC_F64_SPLIT_F16),
+                       ("split_bf16", K.PREC_F64_SPLIT_BF16)]:
+        cfg = K.query(s, s, s, prec, iters=iters)
+        ms = time_ms(lambda: K.block_gemm(A, B, C, precision=prec, iters=iters), steps, warmup)
+        err = float(((C[:32] / iters) - ref).abs().max() / ref.abs().max())
+        tf = flops / ms / 1e9
+        out[name] = {"ms": ms, "tflops": tf, "frac_f64_peak": tf / PEAK["f64"],
+                     "max_rel_err_vs_torch_fp64": err, "partition": cfg["partition"],
+                     "threads": cfg["threads"], "slices": cfg["slices"]}
+    out["split_f16_over_native"] = out["split_f16"]["tflops"] / out["native"]["tflops"]
+    return out
+
+
 # ------------------------------------------------------------------ w4
 
 def uneven_sizes(rng, count, lo=16, hi=256):
