@@ -101,6 +101,9 @@ struct SplitGeom {
     static_assert((BM * 16) % NT == 0 && (BN * 16) % NT == 0, "piece distribution");
     static_assert(LDS <= 160 * 1024, "LDS budget");
     static_assert(levels_exact(BETA, S, G, KC), "slice digits would not sum exactly in FP32");
+    // the persistent form adds the two smallest levels in FP32 before converting them
+    static_assert(G >= 3 && level_bound(BETA, S, G - 1, KC) * pow2(-24 - BETA * (G - 1)) <= pow2(-50),
+                  "the two smallest levels are combined in FP32");
 };
 
 struct SplitArgs {
@@ -475,12 +478,14 @@ split_pipe_kernel(SplitArgs args, int64_t count) {
                     const int ecol = expo[G::BM + pn * G::TN + jn * 16 + (lane & 15)];
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        // smallest weights first
-                        double cs = (double)T[G::G - 2][v] * pow2(-G::BETA * G::G);
+                        // Horner from the smallest weight, in units of level 2; the two
+                        // smallest levels are added in FP32 first (rounding below 2^-50)
+                        double cs = (double)__builtin_fmaf(T[G::G - 2][v], (float)pow2(-G::BETA),
+                                                           T[G::G - 3][v]);
 #pragma unroll
-                        for (int g = G::G - 1; g >= 2; --g)
-                            cs = __builtin_fma((double)T[g - 2][v], pow2(-G::BETA * g), cs);
-                        acc[im][jn][v] += __builtin_amdgcn_ldexp(cs, erow[v] + ecol);
+                        for (int g = G::G - 2; g >= 2; --g)
+                            cs = __builtin_fma(cs, pow2(-G::BETA), (double)T[g - 2][v]);
+                        acc[im][jn][v] += __builtin_amdgcn_ldexp(cs, erow[v] + ecol - 2 * G::BETA);
                     }
                     __builtin_amdgcn_sched_barrier(0);
                 }
