@@ -70,6 +70,30 @@ def test_ragged_shapes_and_empty_rows_exact(prec, bs, dev):
         np.testing.assert_array_equal(Cd.cpu().double().numpy(), ref)
 
 
+@pytest.mark.parametrize("prec", list(PRECS))
+@pytest.mark.parametrize("layout", ["nk", "kn"])
+def test_long_block_rows_exact(prec, layout, dev):
+    """Block rows of 200, 64, 65 and 129 stored blocks: the kernel reads a row's column
+    indices 64 at a time, so these rows cross that chunk boundary 0 to 3 times."""
+    code, dt = PRECS[prec]
+    rng = np.random.default_rng(77)
+    mb, nb, bs, n = 5, 200, 16, 32
+    mask = np.zeros((mb, nb), dtype=bool)
+    mask[0, :] = True
+    for i, cnt in [(1, 64), (2, 65), (4, 129)]:
+        mask[i, np.sort(rng.choice(nb, cnt, replace=False))] = True
+    rp = np.concatenate([[0], np.cumsum(mask.sum(1))]).astype(np.int32)
+    ci = np.nonzero(mask)[1].astype(np.int32)
+    vals = rng.integers(-2, 3, (len(ci), bs, bs)).astype(np.float64)
+    Bnk = rng.integers(-2, 3, (n, nb * bs)).astype(np.float64)
+    Bin = np.ascontiguousarray(Bnk.T) if layout == "kn" else Bnk
+    A = to_dev(rp, ci, vals, mb, nb, bs, dt, dev)
+    C = K.bsr_spmm(A, torch.from_numpy(Bin).to(dt).to(dev), b_layout=layout,
+                   precision=code).cpu().double().numpy()
+    np.testing.assert_array_equal(C, oracle.bsr_spmm(mb, nb, bs, rp, ci, vals, Bin,
+                                                     b_kn=layout == "kn"))
+
+
 def test_padded_leading_dimensions(dev):
     rng = np.random.default_rng(5)
     mb, nb, bs, n = 6, 4, 32, 96
