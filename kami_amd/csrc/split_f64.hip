@@ -369,49 +369,68 @@ split_pipe_kernel(SplitArgs args, int64_t count) {
         c_off = prob * args.sc + (int64_t)tm * G::BM * ldc + (int64_t)tn * G::BN;
     };
 
-    // raw pieces of one chunk: thread t owns 16-byte pieces q = t + i * NT (row q / 16,
-    // k pair q % 16), 16 consecutive lanes per row
-    u4_t xa[G::PA], xb[G::PB];
-    auto load_chunk = [&](int64_t u, int c) {
+    // Slicing layout: thread t owns row t / 4 of A and of B and the 8 consecutive k values
+    // 8 (t mod 4) .. +7 of each chunk (four 16-byte loads from one pointer; one 16-byte
+    // LDS write per digit plane).  The pointers advance by one chunk per step and are set
+    // up once per work unit.
+    static_assert(G::NT == 4 * G::BM && G::NT == 4 * G::BN, "four threads per operand row");
+    const int srow = tid >> 2, sk = 8 * (tid & 3);
+    const double* pa;
+    const double* pb;
+    auto unit_ptrs = [&](int64_t u) {
         int64_t a_off, b_off, c_off;
         unit_base(u, a_off, b_off, c_off);
-        const double* __restrict__ A = args.A + a_off;
-        const double* __restrict__ B = args.B + b_off;
+        pa = args.A + a_off + srow * lda + sk;
+        pb = args.B + b_off + srow * ldb + sk;
+    };
+    u4_t xa[4], xb[4];
+    auto load_chunk = [&]() {
 #pragma unroll
-        for (int i = 0; i < G::PA; ++i) {
-            const int q = tid + i * G::NT;
-            xa[i] = load_raw<16>(A + (q >> 4) * lda + c * G::KC + 2 * (q & 15));
-        }
-#pragma unroll
-        for (int i = 0; i < G::PB; ++i) {
-            const int q = tid + i * G::NT;
-            xb[i] = load_raw<16>(B + (q >> 4) * ldb + c * G::KC + 2 * (q & 15));
+        for (int i = 0; i < 4; ++i) {
+            xa[i] = load_raw<16>(pa + 2 * i);
+            xb[i] = load_raw<16>(pb + 2 * i);
         }
     };
-    // exponent of the row's chunk maximum, reduced over the row's 16 lanes
-    auto row_exp = [&](const u4_t& raw) {
-        const d2_t x = bitcast<d2_t>(raw);
-        return row16_max_exp(max(exp_of(x[0]), exp_of(x[1])));
+    // Exponent e with |x| < 2^e over the row's 32 values of the chunk: the largest upper
+    // word of |x| (monotonic in |x|) over the thread's 8 values and the row's 4 threads (one
+    // DPP quad), completed with all-ones lower bits, so frexp also covers subnormal rows.
+    auto chunk_exp = [&](const u4_t (&x)[4]) {
+        uint32_t h = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h = max(h, max(x[i][1] & 0x7fffffffu, x[i][3] & 0x7fffffffu));
+        h = max(h, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)h, 0xB1, 0xF, 0xF, false));  // quad_perm 1,0,3,2
+        h = max(h, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)h, 0x4E, 0xF, 0xF, false));  // quad_perm 2,3,0,1
+        return __builtin_amdgcn_frexp_exp(__builtin_bit_cast(double, ((uint64_t)h << 32) | 0xffffffffu));
     };
-    auto slice_piece = [&](const u4_t& raw, int row, int kp, int e, unsigned char* plane0) {
-        const d2_t x = bitcast<d2_t>(raw);
-        double r0 = __builtin_amdgcn_ldexp(x[0], G::BETA - e);
-        double r1 = __builtin_amdgcn_ldexp(x[1], G::BETA - e);
+    auto slice = [&](const u4_t (&x)[4], int e, unsigned char* plane0) {
+        unsigned char* dst = plane0 + srow * G::ROWB + 2 * sk;
+        double r[8];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const d2_t d = bitcast<d2_t>(x[i]);
+            r[2 * i] = __builtin_amdgcn_ldexp(d[0], G::BETA - e);
+            r[2 * i + 1] = __builtin_amdgcn_ldexp(d[1], G::BETA - e);
+        }
 #pragma unroll
         for (int s = 0; s < G::S; ++s) {
-            const double q0 = __builtin_rint(r0), q1 = __builtin_rint(r1);
-            *reinterpret_cast<uint32_t*>(plane0 + s * G::PLANE + row * G::ROWB + kp * 4) =
-                pack2<SAT>(q0, q1);
-            if (s + 1 < G::S) {
-                r0 = (r0 - q0) * (double)(1 << G::BETA);
-                r1 = (r1 - q1) * (double)(1 << G::BETA);
+            u4_t packed;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const double q0 = __builtin_rint(r[2 * j]), q1 = __builtin_rint(r[2 * j + 1]);
+                packed[j] = pack2<SAT>(q0, q1);
+                if (s + 1 < G::S) {
+                    r[2 * j] = (r[2 * j] - q0) * (double)(1 << G::BETA);
+                    r[2 * j + 1] = (r[2 * j + 1] - q1) * (double)(1 << G::BETA);
+                }
             }
+            store_raw<16>(dst + s * G::PLANE, packed);
         }
     };
 
     int64_t p = blockIdx.x;  // work unit
     if (p >= count) return;  // uniform
-    load_chunk(p, 0);
+    unit_ptrs(p);
+    load_chunk();
 
     for (; p < count; p += grid) {
         double acc[G::IM][G::JN][4];
@@ -424,25 +443,22 @@ split_pipe_kernel(SplitArgs args, int64_t count) {
 
         for (int c = 0; c < nch; ++c) {
             // ---- exponents and digit planes of this chunk ----
-#pragma unroll
-            for (int i = 0; i < G::PA; ++i) {
-                const int q = tid + i * G::NT, row = q >> 4, kp = q & 15;
-                const int e = row_exp(xa[i]);
-                if (kp == 0) expo[row] = e;
-                slice_piece(xa[i], row, kp, e, smem);
+            const int ea = chunk_exp(xa), eb = chunk_exp(xb);
+            if ((tid & 3) == 0) {
+                expo[srow] = ea;
+                expo[G::BM + srow] = eb;
             }
-#pragma unroll
-            for (int i = 0; i < G::PB; ++i) {
-                const int q = tid + i * G::NT, row = q >> 4, kp = q & 15;
-                const int e = row_exp(xb[i]);
-                if (kp == 0) expo[G::BM + row] = e;
-                slice_piece(xb[i], row, kp, e, smem + G::SA_BYTES);
-            }
+            slice(xa, ea, smem);
+            slice(xb, eb, smem + G::SA_BYTES);
             // the next chunk is in flight during the products
-            if (c + 1 < nch)
-                load_chunk(p, c + 1);
-            else if (p + grid < count)
-                load_chunk(p + grid, 0);
+            if (c + 1 < nch) {
+                pa += G::KC;
+                pb += G::KC;
+                load_chunk();
+            } else if (p + grid < count) {
+                unit_ptrs(p + grid);
+                load_chunk();
+            }
             __syncthreads();
 
             // ---- level products on MFMA, combined in FP64, scaled into acc ----
