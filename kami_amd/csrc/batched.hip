@@ -56,8 +56,8 @@ struct StreamGeom {
     static constexpr int NT = 256;
     static constexpr int ELT = (int)sizeof(typename AT::in_t), VE = 16 / ELT;
     // FP64 stages 16-wide K chunks of tiles up to 64 x 64 (the 128-row variant is not
-    // offered), FP16/BF16 64-wide chunks of tiles up to 128 x 64: 36 KB / 54 KB of LDS for
-    // the two stage buffers, two workgroups per CU
+    // offered), FP16/BF16 64-wide chunks of tiles up to 128 x 64: 40 KB / 60 KB of LDS for
+    // the two stage buffers (160-byte rows), three / two workgroups per CU
     static constexpr int KC = ELT == 8 ? 16 : 64;
     static constexpr int TMAX = ELT == 8 ? 64 : 128, TNMAX = 64;
     static constexpr int ROWB = lds_row_bytes(KC * ELT, AT::FRAG_BYTES);
