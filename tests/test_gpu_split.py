@@ -209,6 +209,39 @@ def test_resident_repeat_alpha_beta_and_adversarial_rows(prec, slices, dev):
     assert np.all(C[0, 0] == 0.0)
 
 
+@pytest.mark.parametrize("prec,slices", RESIDENT, ids=RIDS)
+@pytest.mark.parametrize("s,pipe,iters", [(64, 1, 1), (64, 2, 1), (128, 2, 1), (128, 1, 1),
+                                          (64, 0, 3)])
+def test_padded_leading_dimensions_and_strides(prec, slices, s, pipe, iters, dev):
+    """Rows of A and B padded past K, C rows past N and problems spaced past their spans,
+    through the C ABI, for the one-workgroup, persistent and resident (iters = 3) forms;
+    integer inputs make the result exact and the padding of C must stay untouched."""
+    import ctypes
+    rng = np.random.default_rng(s + pipe + iters)
+    count = 3
+    lda, ldb, ldc = s + 6, s + 10, s + 3
+    sa, sb, sc = s * lda + 32, s * ldb + 16, s * ldc + 5
+    A = rng.integers(-3, 4, size=count * sa).astype(np.float64)
+    B = rng.integers(-3, 4, size=count * sb).astype(np.float64)
+    C = np.full(count * sc, 7.0)
+    Ad, Bd, Cd = (torch.from_numpy(x).to(dev) for x in (A, B, C))
+    d = K.make_desc(s, s, s, prec, slices=slices, iters=iters, pipeline=pipe, lda=lda, ldb=ldb,
+                    ldc=ldc)
+    st = K.lib().kami_block_gemm(ctypes.byref(d), ctypes.c_void_p(Ad.data_ptr()),
+                                 ctypes.c_void_p(Bd.data_ptr()), ctypes.c_void_p(Cd.data_ptr()),
+                                 count, sa, sb, sc,
+                                 ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert st == _abi.OK
+    got = Cd.cpu().numpy()
+    for p in range(count):
+        Ap = A[p * sa: p * sa + s * lda].reshape(s, lda)[:, :s]
+        Bp = B[p * sb: p * sb + s * ldb].reshape(s, ldb)[:, :s]
+        Cp = got[p * sc: p * sc + s * ldc].reshape(s, ldc)
+        np.testing.assert_array_equal(Cp[:, :s], iters * (Ap @ Bp.T))
+        np.testing.assert_array_equal(Cp[:, s:], 7.0)
+        np.testing.assert_array_equal(got[p * sc + s * ldc: (p + 1) * sc], 7.0)
+
+
 def test_split_unsupported_requests_are_enotsup():
     for kw in [dict(b_layout="kn"), dict(partition="3d"), dict(iters=2), dict(slices=3)]:
         with pytest.raises(K.KamiError) as e:
