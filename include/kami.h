@@ -210,6 +210,15 @@ kami_status kami_bsr_spgemm_fill(const kami_bsr* A, const kami_bsr* B, const int
 kami_status kami_bsr_spgemm_numeric(const kami_bsr* A, const kami_bsr* B, const int32_t* rowptrC,
                                     const int32_t* colidxC, int64_t nnzbC, void* valC,
                                     int32_t precision, double alpha, void* stream);
+/* As kami_bsr_spgemm_numeric with the block products repeated `iters` >= 1 times on
+ * operands held in registers, accumulated into valC = alpha * iters * (A * B): the
+ * reference programs' in-kernel NUM_ITER repeat (the loop at src/sparse/spgemm/1d/
+ * block_spgemm_1d_half_mma.cu:192, counted in the FLOP total at :512-516), for measuring
+ * the arithmetic without the memory traffic.  iters = 1 is kami_bsr_spgemm_numeric. */
+kami_status kami_bsr_spgemm_numeric_repeat(const kami_bsr* A, const kami_bsr* B,
+                                           const int32_t* rowptrC, const int32_t* colidxC,
+                                           int64_t nnzbC, void* valC, int32_t precision,
+                                           double alpha, int32_t iters, void* stream);
 
 /* ---------------------------------------------------------------- tuning cache */
 /* kami_block_gemm_autotune and batched plans created with autotune record their choices

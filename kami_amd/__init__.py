@@ -306,20 +306,29 @@ def spgemm_fill(A: BsrMatrix, B: BsrMatrix, rowptrC, nnzbC, stream=None):
 
 
 def spgemm_numeric(A: BsrMatrix, B: BsrMatrix, rowptrC, colidxC, *, precision=None, alpha=1.0,
-                   stream=None):
-    """Values of C = alpha * A * B on the symbolic structure: (nnzbC, bs, bs), float32 for
-    FP16/BF16 inputs, float64 for FP64."""
+                   iters=1, stream=None):
+    """Values of C = alpha * iters * A * B on the symbolic structure: (nnzbC, bs, bs),
+    float32 for FP16/BF16 inputs, float64 for FP64.  iters > 1 repeats the block products
+    inside the kernel (kami_bsr_spgemm_numeric_repeat)."""
     import torch
     prec = _precision_of(A.vals, precision)
     out_dt = torch.float64 if prec == PREC_F64 else torch.float32
     nnz = int(colidxC.numel())
     vals = torch.empty((max(nnz, 1), A.bs, A.bs), dtype=out_dt, device=A.rowptr.device)
     a, b = A._c(), B._c()
-    check(lib().kami_bsr_spgemm_numeric(ctypes.byref(a), ctypes.byref(b),
-                                        ctypes.c_void_p(rowptrC.data_ptr()),
-                                        ctypes.c_void_p(colidxC.data_ptr() if nnz else 0), nnz,
-                                        ctypes.c_void_p(vals.data_ptr()), prec, alpha,
-                                        _stream_ptr(stream)), "kami_bsr_spgemm_numeric")
+    if iters == 1:
+        check(lib().kami_bsr_spgemm_numeric(ctypes.byref(a), ctypes.byref(b),
+                                            ctypes.c_void_p(rowptrC.data_ptr()),
+                                            ctypes.c_void_p(colidxC.data_ptr() if nnz else 0), nnz,
+                                            ctypes.c_void_p(vals.data_ptr()), prec, alpha,
+                                            _stream_ptr(stream)), "kami_bsr_spgemm_numeric")
+    else:
+        check(lib().kami_bsr_spgemm_numeric_repeat(ctypes.byref(a), ctypes.byref(b),
+                                                   ctypes.c_void_p(rowptrC.data_ptr()),
+                                                   ctypes.c_void_p(colidxC.data_ptr() if nnz else 0),
+                                                   nnz, ctypes.c_void_p(vals.data_ptr()), prec,
+                                                   alpha, int(iters), _stream_ptr(stream)),
+              "kami_bsr_spgemm_numeric_repeat")
     return vals[:nnz]
 
 

@@ -99,6 +99,9 @@ _PROTOTYPES = {
     "kami_bsr_spgemm_numeric": (c_int32, [ctypes.POINTER(Bsr), ctypes.POINTER(Bsr), c_void_p,
                                           c_void_p, c_int64, c_void_p, c_int32, c_double,
                                           c_void_p]),
+    "kami_bsr_spgemm_numeric_repeat": (c_int32, [ctypes.POINTER(Bsr), ctypes.POINTER(Bsr),
+                                                 c_void_p, c_void_p, c_int64, c_void_p, c_int32,
+                                                 c_double, c_int32, c_void_p]),
     "kami_tune_save": (c_int32, [ctypes.c_char_p]),
     "kami_tune_load": (c_int32, [ctypes.c_char_p]),
     "kami_tune_clear": (c_int32, []),

@@ -177,7 +177,8 @@ template <int BS, class AT>
 __global__ void __launch_bounds__(64)
 spgemm_numeric_kernel(int mb, int64_t nnzc, const int32_t* rpA, const int32_t* ciA,
                       const void* vA_, const int32_t* rpB, const int32_t* ciB, const void* vB_,
-                      const int32_t* rpC, const int32_t* ciC, void* vC_, double alpha_d) {
+                      const int32_t* rpC, const int32_t* ciC, void* vC_, double alpha_d,
+                      int iters) {
     using G = SpNumGeom<BS, AT>;
     using in_t = typename AT::in_t;
     using acc_t = typename AT::acc_t;
@@ -246,22 +247,27 @@ spgemm_numeric_kernel(int mb, int64_t nnzc, const int32_t* rpA, const int32_t* c
                     *reinterpret_cast<in_t*>(sB + (jc + x) * G::ROWB + kr * G::ELT) = e[x];
             }
             __syncthreads();
+            // both blocks' fragments in registers; `iters` > 1 repeats the products on them
+            // (the reference's in-kernel NUM_ITER repeat), iters = 1 is the plain product
+            frag_t af[G::KSTEPS][G::IM], bf[G::KSTEPS][G::IM];
 #pragma unroll
             for (int ks = 0; ks < G::KSTEPS; ++ks) {
                 const int kk = ks * AT::KSTEP + AT::KV * (lane >> 4);
-                frag_t af[G::IM];
 #pragma unroll
-                for (int im = 0; im < G::IM; ++im)
-                    af[im] = bitcast<frag_t>(load_raw<FB>(Ablk + (im * 16 + (lane & 15)) * BS + kk));
-#pragma unroll
-                for (int jn = 0; jn < G::IM; ++jn) {
-                    const frag_t bf = bitcast<frag_t>(
-                        load_raw<FB>(sB + (jn * 16 + (lane & 15)) * G::ROWB + kk * G::ELT));
-#pragma unroll
-                    for (int im = 0; im < G::IM; ++im) AT::mma(acc[im][jn], af[im], bf);
+                for (int x = 0; x < G::IM; ++x) {
+                    af[ks][x] = bitcast<frag_t>(load_raw<FB>(Ablk + (x * 16 + (lane & 15)) * BS + kk));
+                    bf[ks][x] = bitcast<frag_t>(
+                        load_raw<FB>(sB + (x * 16 + (lane & 15)) * G::ROWB + kk * G::ELT));
                 }
             }
             __syncthreads();
+            for (int it = 0; it < iters; ++it)
+#pragma unroll
+                for (int ks = 0; ks < G::KSTEPS; ++ks)
+#pragma unroll
+                    for (int jn = 0; jn < G::IM; ++jn)
+#pragma unroll
+                        for (int im = 0; im < G::IM; ++im) AT::mma(acc[im][jn], af[ks][im], bf[ks][jn]);
           }
         }
         out_t* Cblk = vC + cb * BS * BS;
@@ -356,6 +362,15 @@ kami_status kami_bsr_spgemm_fill(const kami_bsr* A, const kami_bsr* B, const int
 kami_status kami_bsr_spgemm_numeric(const kami_bsr* A, const kami_bsr* B, const int32_t* rowptrC,
                                     const int32_t* colidxC, int64_t nnzbC, void* valC,
                                     int32_t precision, double alpha, void* stream) {
+    return kami_bsr_spgemm_numeric_repeat(A, B, rowptrC, colidxC, nnzbC, valC, precision, alpha,
+                                          1, stream);
+}
+
+kami_status kami_bsr_spgemm_numeric_repeat(const kami_bsr* A, const kami_bsr* B,
+                                           const int32_t* rowptrC, const int32_t* colidxC,
+                                           int64_t nnzbC, void* valC, int32_t precision,
+                                           double alpha, int32_t iters, void* stream) {
+    if (iters < 1) return KAMI_EINVAL;
     kami_status st = check_pair(A, B);
     if (st != KAMI_OK) return st;
     if (nnzbC < 0 || !rowptrC) return KAMI_EINVAL;
@@ -371,7 +386,7 @@ kami_status kami_bsr_spgemm_numeric(const kami_bsr* A, const kami_bsr* B, const 
 #define KAMI_SPNUM(BS, AT)                                                                      \
     hipLaunchKernelGGL((spgemm_numeric_kernel<BS, AT>), dim3(grid), dim3(64), 0, s, A->mb,       \
                        nnzbC, A->rowptr, A->colidx, A->vals, B->rowptr, B->colidx, B->vals,      \
-                       rowptrC, colidxC, valC, alpha)
+                       rowptrC, colidxC, valC, alpha, (int)iters)
     if (precision == KAMI_PREC_F16_F32) {
         if (A->bs == 16) KAMI_SPNUM(16, AtomF16x16); else KAMI_SPNUM(32, AtomF16x32);
     } else if (precision == KAMI_PREC_BF16_F32) {

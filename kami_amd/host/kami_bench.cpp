@@ -369,6 +369,8 @@ int run_spgemm(const Args& a) {
     }
     const int nbr = (int)a.geti("--blocks", 8), bs = (int)a.geti("--bs", 16);
     const float sparsity = (float)a.getd("--sparsity", 0.5);
+    // NUM_ITER: the numeric phase repeats its block products in the kernel
+    const int iters = (int)a.geti("--iters", 1);
     print_device();
     std::vector<int32_t> rp, ci;
     std::vector<double> val;
@@ -403,9 +405,11 @@ int run_spgemm(const Args& a) {
     const size_t eo = half_input(prec) ? 4 : 8;
     void* dvalC = nullptr;
     HIP_OK(hipMalloc(&dvalC, std::max<int64_t>(nnzbC, 1) * bs * bs * eo));
-    KAMI_OK_OR_DIE(kami_bsr_spgemm_numeric(&A, &A, drpC, dciC, nnzbC, dvalC, prec, 1.0, nullptr));
+    KAMI_OK_OR_DIE(kami_bsr_spgemm_numeric_repeat(&A, &A, drpC, dciC, nnzbC, dvalC, prec, 1.0,
+                                                  iters, nullptr));
     HIP_OK(hipEventRecord(t1, nullptr));
-    KAMI_OK_OR_DIE(kami_bsr_spgemm_numeric(&A, &A, drpC, dciC, nnzbC, dvalC, prec, 1.0, nullptr));
+    KAMI_OK_OR_DIE(kami_bsr_spgemm_numeric_repeat(&A, &A, drpC, dciC, nnzbC, dvalC, prec, 1.0,
+                                                  iters, nullptr));
     HIP_OK(hipEventRecord(t2, nullptr));
     HIP_OK(hipEventSynchronize(t2));
     float ms = 0;
@@ -413,7 +417,7 @@ int run_spgemm(const Args& a) {
     long long nnzCub = 0;  // sum over stored blocks (i, k) of A of the blocks in B's row k
     for (int i = 0; i < nbr; ++i)
         for (int q = rp[i]; q < rp[i + 1]; ++q) nnzCub += rp[ci[q] + 1] - rp[ci[q]];
-    const double ops = 2.0 * (double)nnzCub * bs * bs * bs;
+    const double ops = 2.0 * (double)nnzCub * bs * bs * bs * iters;
     const double gflops = ops / (ms * 1e6), tflops = gflops / 1e3;
     std::printf("Execution time: %g ms\n", ms);
     std::printf("Performance: %g GFLOPS (%g TFLOPS)\n", gflops, tflops);
@@ -453,7 +457,7 @@ int run_spgemm(const Args& a) {
                 break;
             }
             for (int e = 0; e < bs * bs; ++e)
-                if (std::fabs(C[(size_t)pos * bs * bs + e] - acc[(size_t)j * bs * bs + e]) > 1e-6) {
+                if (std::fabs(C[(size_t)pos * bs * bs + e] / iters - acc[(size_t)j * bs * bs + e]) > 1e-6) {
                     std::printf("Error: block (%d, %d) element %d\n", i, j, e);
                     error = 1;
                     break;
@@ -570,7 +574,7 @@ void usage() {
         "       kami_bench batched [--prec f64|f16] [--m M --n N --k K] [--count C]\n"
         "                          [--iters I] [--graph 0|1] [--autotune 0|1]\n"
         "       kami_bench spgemm  [--prec f16|bf16|f64] [--blocks B] [--bs 16|32]\n"
-        "                          [--sparsity S]\n"
+        "                          [--sparsity S] [--iters I]\n"
         "       kami_bench spmm    [--prec f16|bf16|f64] [--blocks B] [--bs 16|32] [--n N]\n"
         "                          [--sparsity S]\n");
 }

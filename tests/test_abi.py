@@ -120,6 +120,13 @@ def test_split_precision_query_and_enotsup():
         assert e.value.status == _abi.ENOTSUP
 
 
+def test_spgemm_numeric_repeat_rejects_iters_below_one():
+    bsr = _abi.Bsr()
+    assert K.lib().kami_bsr_spgemm_numeric_repeat(ctypes.byref(bsr), ctypes.byref(bsr), None,
+                                                  None, 0, None, K.PREC_F16_F32, 1.0, 0,
+                                                  None) == _abi.EINVAL
+
+
 def test_batched_plan_validation_without_device():
     arr = (_abi.Problem * 2)()
     for p in arr:

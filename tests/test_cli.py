@@ -46,6 +46,8 @@ CASES = [
     (["batched", "--prec", "f16", "--m", "48", "--n", "80", "--k", "112", "--count", "200", "--graph", "0"],
      r"\[hemeng_log\],1d,48,80,112,200,"),
     (["spgemm", "--prec", "f16", "--blocks", "8", "--bs", "16"], r"\[hemeng_log\],2d,\d+,128,128,128,"),
+    (["spgemm", "--prec", "f16", "--blocks", "8", "--bs", "16", "--iters", "1000"],
+     r"\[hemeng_log\],2d,\d+,128,128,128,"),
     (["spgemm", "--prec", "f64", "--blocks", "24", "--bs", "32", "--sparsity", "0.8"],
      r"\[hemeng_log\],2d,\d+,768,768,768,"),
     (["spmm", "--prec", "f16", "--blocks", "8", "--bs", "16"], r"\[hemeng_log\],2d,128,4,"),

@@ -65,6 +65,24 @@ def test_reference_default_input_exact(dev):
     np.testing.assert_array_equal(C.astype(np.float64), ref)
 
 
+@pytest.mark.parametrize("bs,dt", [(16, torch.float16), (16, torch.bfloat16), (32, torch.float16),
+                                   (16, torch.float64), (32, torch.float64)],
+                         ids=["f16-16", "bf16-16", "f16-32", "f64-16", "f64-32"])
+def test_in_kernel_repeat_is_iters_times_the_product(bs, dt, dev):
+    """kami_bsr_spgemm_numeric_repeat (the reference's NUM_ITER in-kernel repeat): on the
+    reference generator's integer values every repetition is exact, so C(iters) equals
+    iters * C(1) and iters * the oracle."""
+    rp, ci, vals, _ = oracle.spmm_generate(8, 8, 0.5, bs, 16)
+    A = to_bsr(8, 8, bs, rp, ci, vals, dev, dtype=dt)
+    rowptr, colidx, rp_ref, nnz = check_structure(A, A, rp, ci, rp, ci)
+    ref = oracle.spgemm_numeric(8, 8, bs, rp, ci, vals, rp, ci, vals, rp_ref)
+    one = K.spgemm_numeric(A, A, rowptr, colidx).cpu().double().numpy()
+    np.testing.assert_array_equal(one, ref)
+    for iters in (2, 7):
+        got = K.spgemm_numeric(A, A, rowptr, colidx, iters=iters, alpha=0.5).cpu().double().numpy()
+        np.testing.assert_array_equal(got, 0.5 * iters * ref)
+
+
 def test_large_2pct_32x32_blocks(dev):
     """C = A * A, 8192 x 8192 with 32 x 32 blocks at 2 % block density, N(0,1) FP16."""
     rng = np.random.default_rng(0x4B414D49 + 4)
