@@ -9,10 +9,10 @@
 //     [16 w, 16 w + 16) of the tile and all bs rows of the block row;
 //   * only the A block is shared by the four waves, so only it is staged in LDS (rows
 //     padded to the conflict-free stride, double-buffered, one barrier per block); each
-//     lane loads its own B values straight into MFMA fragments: lane (c, g) holds column c
-//     and the contiguous k range [g bs/4, (g+1) bs/4) of the block, and the A block is
-//     written to LDS with its columns permuted to match, so the products are those of the
-//     block's k values in another order;
+//     lane loads its own B values straight into MFMA fragments, in the fragment layout of
+//     the MFMA atom (for column c, k-step t and lane group g the KV values from
+//     t KSTEP + g KV), so one load instruction reads 4 x 16 contiguous bytes of every
+//     column it touches;
 //   * the next block's A piece and B fragments are loaded while the current block is
 //     multiplied; the column indices of a row are read 64 at a time into a register and
 //     taken out with readlane, so no scalar load waits in the loop;
@@ -40,15 +40,8 @@ struct SpmmGeom {
     static constexpr int NPA = (NA + NT - 1) / NT;
     static constexpr int IM = BS / 16;
     static constexpr int NKS = BS / AT::KSTEP;  // MFMA k-steps per block
-    static constexpr int KPL = BS / 4;          // k values of B one lane holds per block
     static_assert(BS % AT::KSTEP == 0 && BS % 16 == 0, "block size");
-    static_assert(KPL == NKS * AT::KV, "four lane groups split the block's k range");
-    static_assert(FB == 16 || NKS == 1, "the k permutation moves 16-byte fragments");
     static_assert(TN == 16 * (NT / 64), "one 16-column atom per wave");
-    // LDS piece of A-row piece j: piece j holds k values [j KV, j KV + KV) = lane group
-    // g = j / NKS, k-step t = j % NKS, which the fragment read of step t, group g finds at
-    // piece 4 t + g.
-    __device__ static constexpr int a_piece(int j) { return (j % NKS) * 4 + j / NKS; }
 };
 
 struct SpmmArgs {
@@ -119,10 +112,10 @@ __global__ void __launch_bounds__(256) spmm_kernel(SpmmArgs a) {
         for (int p = 0; p < G::NPA; ++p) {
             const int idx = tid + p * G::NT;
             if (idx < G::NA)
-                store_raw<16>(base + (idx / G::CPR) * G::ROWB + G::a_piece(idx % G::CPR) * 16, ra[p]);
+                store_raw<16>(base + (idx / G::CPR) * G::ROWB + (idx % G::CPR) * 16, ra[p]);
         }
     };
-    // B fragments of block column kb for this lane: step t holds k = kb bs + g KPL + t KV + v
+    // B fragments of block column kb for this lane: step t holds k = kb bs + t KSTEP + g KV + v
     auto load_b = [&](int kb, frag_t (&f)[G::NKS]) {
         elems_t z;
 #pragma unroll
@@ -130,18 +123,18 @@ __global__ void __launch_bounds__(256) spmm_kernel(SpmmArgs a) {
 #pragma unroll
         for (int t = 0; t < G::NKS; ++t) f[t] = bitcast<frag_t>(z);
         if (!wave_live || kb < 0 || kb >= a.nb) return;  // wave-uniform
-        const int64_t k0 = (int64_t)kb * BS + grp * G::KPL;
+        const int64_t k0 = (int64_t)kb * BS + grp * KV;
         if (!a.b_kn) {
             const in_t* p = B + (int64_t)col * a.ldb + k0;
 #pragma unroll
-            for (int t = 0; t < G::NKS; ++t) f[t] = bitcast<frag_t>(load_raw<FB>(p + t * KV));
+            for (int t = 0; t < G::NKS; ++t) f[t] = bitcast<frag_t>(load_raw<FB>(p + t * AT::KSTEP));
         } else {
             const in_t* p = B + k0 * a.ldb + col;
 #pragma unroll
             for (int t = 0; t < G::NKS; ++t) {
                 elems_t e;
 #pragma unroll
-                for (int v = 0; v < KV; ++v) e.v[v] = p[(int64_t)(t * KV + v) * a.ldb];
+                for (int v = 0; v < KV; ++v) e.v[v] = p[(int64_t)(t * AT::KSTEP + v) * a.ldb];
                 f[t] = bitcast<frag_t>(e);
             }
         }
