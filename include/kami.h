@@ -82,8 +82,8 @@ typedef struct kami_gemm_desc {
     int32_t b_layout;  /* kami_b_layout */
     int32_t iters;     /* >= 1: the product is accumulated `iters` times before the
                           epilogue, as the reference's NUM_ITER repeat does; 1 = plain GEMM.
-                          Split precision takes iters > 1 at 64^3 (resident form: digits
-                          made once, the slice products and recombination repeated) */
+                          Split precision takes iters > 1 at 32^3 and 64^3 (resident form:
+                          digits made once, the slice products and recombination repeated) */
     int64_t lda, ldb, ldc; /* elements; 0 = packed */
     double alpha, beta;    /* C = alpha * (A B) + beta * C */
     int32_t slices;        /* split precision: slices per operand, 0 = default */

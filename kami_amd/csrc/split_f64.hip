@@ -805,6 +805,8 @@ static const std::vector<SplitInstance>& split_registry() {
         // FP16 with an 8-bit first digit, so that every level sum runs over all of K
         add_split_resident<AtomF16x32, F16, 64, 64, 64, 4, 2, 9, 6, 7, 8>(r);
         add_split_resident<AtomBF16x32, BF, 64, 64, 64, 4, 2, 8, 7, 8, 8>(r);
+        add_split_resident<AtomF16x32, F16, 32, 32, 32, 2, 2, 9, 6, 7, 8>(r);
+        add_split_resident<AtomBF16x32, BF, 32, 32, 32, 2, 2, 8, 7, 8, 8>(r);
         return r;
     }();
     return v;

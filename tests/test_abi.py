@@ -118,6 +118,11 @@ def test_split_precision_query_and_enotsup():
         with pytest.raises(K.KamiError) as e:
             K.query(64, 64, 64, prec, iters=3, pipeline=2)
         assert e.value.status == _abi.ENOTSUP
+        # 32^3 exists only as a resident form
+        assert K.query(32, 32, 32, prec, iters=2)["k_stage"] == 32
+        with pytest.raises(K.KamiError) as e:
+            K.query(32, 32, 32, prec, iters=1)
+        assert e.value.status == _abi.ENOTSUP
 
 
 def test_spgemm_numeric_repeat_rejects_iters_below_one():

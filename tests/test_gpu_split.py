@@ -149,20 +149,22 @@ def test_persistent_walks_many_problems(prec, slices, dev):
         np.testing.assert_array_equal(C, oracle.gemm_f64(A, B))
 
 
-# resident-repeat forms (iters > 1): 64^3, the default slice count of each digit type
+# resident-repeat forms (iters > 1): 32^3 and 64^3, the default slice count of each
+# digit type
 RESIDENT = [(K.PREC_F64_SPLIT_F16, 6), (K.PREC_F64_SPLIT_BF16, 7)]
 RIDS = ["f16-s6", "bf16-s7"]
 
 
 @pytest.mark.parametrize("prec,slices", RESIDENT, ids=RIDS)
 @pytest.mark.parametrize("iters", [2, 5])
-def test_resident_repeat_kat_exact(prec, slices, iters, dev):
+@pytest.mark.parametrize("s", [32, 64])
+def test_resident_repeat_kat_exact(prec, slices, iters, s, dev):
     """The resident form repeats the slice products `iters` times into one accumulator, as
     the reference's NUM_ITER loop does; on integer inputs every digit, level sum and FP64
     addition is exact, so C = iters * A B exactly (ramp and rand_r % 3 inputs)."""
-    cfg = K.query(64, 64, 64, prec, slices=slices, iters=iters)
-    assert cfg["k_stage"] == 64 and cfg["pipeline"] == 1 and cfg["prefetch"] == 0
-    s, count = 64, 3
+    cfg = K.query(s, s, s, prec, slices=slices, iters=iters)
+    assert cfg["k_stage"] == s and cfg["pipeline"] == 1 and cfg["prefetch"] == 0
+    count = 3
     ramp = np.tile(oracle.ramp(s * s).reshape(1, s, s), (count, 1, 1))
     randr = oracle.randr_mod3(count * s * s).reshape(count, s, s) - 1.0
     for A, B in [(ramp, ramp), (randr, ramp), (randr, randr * 7.0)]:
@@ -171,24 +173,28 @@ def test_resident_repeat_kat_exact(prec, slices, iters, dev):
 
 
 @pytest.mark.parametrize("prec,slices", RESIDENT, ids=RIDS)
-def test_resident_repeat_log_uniform_normwise_1e13(prec, slices, dev):
-    rng = np.random.default_rng(29)
-    s, count, iters = 64, 64, 3
+@pytest.mark.parametrize("s", [32, 64])
+def test_resident_repeat_log_uniform_normwise_1e13(prec, slices, s, dev):
+    rng = np.random.default_rng(29 + s)
+    count, iters = 64, 3
     A = log_uniform(rng, (count, s, s))
     B = log_uniform(rng, (count, s, s))
     C = run(A, B, prec, slices, dev, iters=iters)
     exact, _ = oracle.gemm_dot2(A, B)
     assert normwise(C / iters, exact) <= 1e-13
-    # the one-workgroup-per-problem form (iters = 1) agrees far inside the bound; the
-    # resident FP16 form slices with an 8-bit first digit, so the two differ by a few ulps
-    C1 = run(A, B, prec, slices, dev, pipe=1)
-    assert normwise(C / iters, C1) <= 1e-14
+    # the one-workgroup-per-problem form (iters = 1, 64^3 only) agrees far inside the
+    # bound; the resident FP16 form slices with an 8-bit first digit, so the two differ
+    # by a few ulps
+    if s == 64:
+        C1 = run(A, B, prec, slices, dev, pipe=1)
+        assert normwise(C / iters, C1) <= 1e-14
 
 
 @pytest.mark.parametrize("prec,slices", RESIDENT, ids=RIDS)
-def test_resident_repeat_alpha_beta_and_adversarial_rows(prec, slices, dev):
-    rng = np.random.default_rng(31)
-    s, iters = 64, 2
+@pytest.mark.parametrize("s", [32, 64])
+def test_resident_repeat_alpha_beta_and_adversarial_rows(prec, slices, s, dev):
+    rng = np.random.default_rng(31 + s)
+    iters = 2
     A = rng.integers(-4, 5, (2, s, s)).astype(np.float64)
     B = rng.integers(-4, 5, (2, s, s)).astype(np.float64)
     C0 = rng.integers(-9, 10, (2, s, s)).astype(np.float64)
@@ -211,7 +217,7 @@ def test_resident_repeat_alpha_beta_and_adversarial_rows(prec, slices, dev):
 
 @pytest.mark.parametrize("prec,slices", RESIDENT, ids=RIDS)
 @pytest.mark.parametrize("s,pipe,iters", [(64, 1, 1), (64, 2, 1), (128, 2, 1), (128, 1, 1),
-                                          (64, 0, 3)])
+                                          (64, 0, 3), (32, 0, 3)])
 def test_padded_leading_dimensions_and_strides(prec, slices, s, pipe, iters, dev):
     """Rows of A and B padded past K, C rows past N and problems spaced past their spans,
     through the C ABI, for the one-workgroup, persistent and resident (iters = 3) forms;
