@@ -77,7 +77,20 @@ def cpu_baseline_f64(m, n, k, count, budget_s=10.0):
     tflops = 2.0 * m * n * k * count * reps / dt / 1e12
     return {"value": tflops, "unit": "TFLOPS", "cores": cores, "kind": "port",
             "sample": f"{reps} passes over all {count} problems {m}x{n}x{k} FP64 (oracle "
-                      f"triple loop, OpenMP over problems, {cores} threads), {dt:.1f} s"}
+                      f"triple loop, OpenMP over problems, {cores} threads), {dt:.1f} s",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
+
+
+def cpu_model():
+    """The host CPU's model name from /proc/cpuinfo (None where it is not readable)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def measured_traffic(path, cfg, m, count):

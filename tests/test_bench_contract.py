@@ -43,6 +43,11 @@ def test_pmc_records_are_quoted_only_for_their_instance_and_workload():
     assert b.measured_lds("native", cfg, 128, 1024) is None
 
 
+def test_cpu_model_is_read_from_the_host():
+    model = _bench_module().cpu_model()
+    assert model is None or (isinstance(model, str) and model)
+
+
 def test_graft_entry_points_exist():
     spec = importlib.util.spec_from_file_location("graft_entry",
                                                   os.path.join(ROOT, "__graft_entry__.py"))
