@@ -1,5 +1,5 @@
 """The benchmark's contract: one GPU only, the PMC records it quotes, and (on a GPU) one
-JSON line with the fields the driver reads."""
+JSON line with the fields a caller of bench.py reads."""
 import importlib.util
 import json
 import os
