@@ -201,6 +201,11 @@ def test_spmm_validation_without_device():
     assert call() == _abi.ENOTSUP
     A.bs = 16
     assert call() == _abi.ENODEV              # valid request, no GPU here
+    # one workgroup per (block row, 64 columns): 2^24 of them is beyond one dispatch
+    A.mb = 1 << 22
+    assert call(n=256) == _abi.ENOTSUP        # 2^22 x 4 = 2^24
+    assert call(n=192) == _abi.ENODEV         # 2^22 x 3 fits
+    A.mb = 2
 
 
 def test_split_gemm_validation_without_device():
