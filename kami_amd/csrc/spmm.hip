@@ -11,8 +11,8 @@
 //     padded to the conflict-free stride, double-buffered, one barrier per block); each
 //     lane loads its own B values straight into MFMA fragments, in the fragment layout of
 //     the MFMA atom (for column c, k-step t and lane group g the KV values from
-//     t KSTEP + g KV), so one load instruction reads 4 x 16 contiguous bytes of every
-//     column it touches;
+//     t KSTEP + g KV), so one load instruction reads four consecutive fragments (64 bytes
+//     with 16-byte fragments) of every column it touches;
 //   * the next block's A piece and B fragments are loaded while the current block is
 //     multiplied; the column indices of a row are read 64 at a time into a register and
 //     taken out with readlane, so no scalar load waits in the loop;
