@@ -156,7 +156,10 @@ kami_status kami_gemm_desc_init(kami_gemm_desc* d, int32_t m, int32_t n, int32_t
 /* Validate a descriptor and report the instance it resolves to. Touches no device. */
 kami_status kami_block_gemm_query(const kami_gemm_desc* d, kami_gemm_config* out);
 /* count independent problems; problem p uses A + p*stride_a, B + p*stride_b,
- * C + p*stride_c (elements; 0 = packed: m*k, n*k, m*n). One problem per workgroup. */
+ * C + p*stride_c (elements; 0 = packed: m*k, n*k, m*n). One problem per workgroup.
+ * KAMI_EINVAL when count > 1 and a nonzero stride_c is below the span of one C
+ * (the problems' outputs would overlap); KAMI_ENOTSUP when count workgroups of the
+ * chosen instance would reach 2^32 work-items. */
 kami_status kami_block_gemm(const kami_gemm_desc* d, const void* A, const void* B, void* C,
                             int64_t count, int64_t stride_a, int64_t stride_b,
                             int64_t stride_c, void* stream);
@@ -188,7 +191,11 @@ kami_status kami_split_gemm(const kami_gemm_desc* d, const void* A, const void* 
 kami_status kami_batched_plan_create(const kami_problem* probs, int64_t count,
                                      int32_t precision, const kami_batched_opts* opts,
                                      void* stream, kami_batched_plan** out);
-/* One persistent launch (through the plan's hipGraph when use_graph is set). */
+/* One persistent launch (through the plan's hipGraph when use_graph is set).
+ * A plan owns one device queue counter that every execution resets and then draws
+ * from, so a plan allows one execution in flight: serialise its executions on one
+ * stream (a second stream, or two replays of a captured graph running at once, would
+ * race on the counter and skip or repeat tiles).  Distinct plans run concurrently. */
 kami_status kami_batched_execute(kami_batched_plan* plan, void* stream);
 kami_status kami_batched_plan_info(const kami_batched_plan* plan, kami_plan_info* out);
 kami_status kami_batched_plan_destroy(kami_batched_plan* plan);

@@ -77,6 +77,39 @@ def _precision_of(A, precision):
     raise TypeError(f"unsupported input dtype {A.dtype}")
 
 
+_IN_DTYPE = {}  # precision -> torch dtype of A and B, filled on first use
+
+
+def _in_dtype(prec):
+    import torch
+    if not _IN_DTYPE:
+        _IN_DTYPE.update({PREC_F64: torch.float64, PREC_F64_SPLIT_F16: torch.float64,
+                          PREC_F64_SPLIT_BF16: torch.float64, PREC_F16_F32: torch.float16,
+                          PREC_BF16_F32: torch.bfloat16})
+    if prec not in _IN_DTYPE:
+        raise ValueError(f"unknown precision {prec}")
+    return _IN_DTYPE[prec]
+
+
+def _out_dtype(prec):
+    import torch
+    return torch.float32 if prec in (PREC_F16_F32, PREC_BF16_F32) else torch.float64
+
+
+def _expect(t, name, shape, dtype, device):
+    """Raise ValueError unless tensor t has exactly this shape, dtype and device and is
+    contiguous: the library receives raw pointers and would read or write past a tensor
+    that does not match."""
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if t.device != device:
+        raise ValueError(f"{name} is on {t.device}, expected {device}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
 def _stream_ptr(stream):
     import torch
     if stream is None:
@@ -98,18 +131,18 @@ def block_gemm(A, B, C=None, *, precision=None, partition="auto", ranks=0, b_lay
         A, B = A.unsqueeze(0), B.unsqueeze(0)
         if C is not None:
             C = C.unsqueeze(0)
+    if A.dim() != 3 or B.dim() != 3:
+        raise ValueError("block_gemm expects A (count, M, K) and B (count, N, K) or (count, K, N)")
     count, M, K = A.shape
     lay = _layout(b_layout)
     N = B.shape[1] if lay == B_NK else B.shape[2]
     prec = _precision_of(A, precision)
-    out_dtype = torch.float32 if prec in (PREC_F16_F32, PREC_BF16_F32) else torch.float64
+    out_dtype = _out_dtype(prec)
     if C is None:
         C = torch.empty((count, M, N), dtype=out_dtype, device=A.device)
-    for t in (A, B, C):
-        if not t.is_contiguous():
-            raise ValueError("block_gemm expects contiguous tensors")
-    if C.dtype != out_dtype or tuple(C.shape) != (count, M, N):
-        raise ValueError(f"C must be {out_dtype} of shape {(count, M, N)}")
+    _expect(A, "A", (count, M, K), _in_dtype(prec), A.device)
+    _expect(B, "B", (count, N, K) if lay == B_NK else (count, K, N), _in_dtype(prec), A.device)
+    _expect(C, "C", (count, M, N), out_dtype, A.device)
     d = make_desc(M, N, K, prec, partition=partition, ranks=ranks, b_layout=lay, iters=iters,
                   alpha=alpha, beta=beta, slices=slices, pipeline=pipeline)
     st = lib().kami_block_gemm(ctypes.byref(d), ctypes.c_void_p(A.data_ptr()),
@@ -132,13 +165,15 @@ def split_gemm(A, B, C=None, *, precision=PREC_F64_SPLIT_F16, slices=0, alpha=1.
         A, B = A.unsqueeze(0), B.unsqueeze(0)
         if C is not None:
             C = C.unsqueeze(0)
+    if A.dim() != 3 or B.dim() != 3:
+        raise ValueError("split_gemm expects A (count, M, K) and B (count, N, K)")
     count, M, K = A.shape
     N = B.shape[1]
     if C is None:
         C = torch.empty((count, M, N), dtype=torch.float64, device=A.device)
-    for t in (A, B, C):
-        if not t.is_contiguous() or t.dtype != torch.float64:
-            raise ValueError("split_gemm expects contiguous float64 tensors")
+    _expect(A, "A", (count, M, K), torch.float64, A.device)
+    _expect(B, "B", (count, N, K), torch.float64, A.device)
+    _expect(C, "C", (count, M, N), torch.float64, A.device)
     d = make_desc(M, N, K, precision, alpha=alpha, beta=beta, slices=slices)
     check(lib().kami_split_gemm(ctypes.byref(d), ctypes.c_void_p(A.data_ptr()),
                                 ctypes.c_void_p(B.data_ptr()), ctypes.c_void_p(C.data_ptr()),
@@ -205,9 +240,15 @@ class BatchedPlan:
         prec = _precision_of(A0, precision)
         lay = _layout(b_layout)
         arr = (_abi.Problem * len(problems))()
+        in_dt, out_dt = _in_dtype(prec), _out_dtype(prec)
         for i, (A, B, C) in enumerate(problems):
+            if A.dim() != 2 or B.dim() != 2 or C.dim() != 2:
+                raise ValueError(f"problem {i}: A, B and C must be 2-D")
             m, k = A.shape
             n = B.shape[0] if lay == B_NK else B.shape[1]
+            _expect(A, f"problem {i} A", (m, k), in_dt, A0.device)
+            _expect(B, f"problem {i} B", (n, k) if lay == B_NK else (k, n), in_dt, A0.device)
+            _expect(C, f"problem {i} C", (m, n), out_dt, A0.device)
             p = arr[i]
             p.m, p.n, p.k = m, n, k
             p.lda = p.ldb = p.ldc = 0
@@ -270,6 +311,22 @@ class BsrMatrix:
     def nnzb(self):
         return int(self.colidx.numel())
 
+    def check(self, name="matrix"):
+        """Raise ValueError unless the arrays match the sizes: rowptr int32 (mb + 1,),
+        colidx int32 (nnzb,), vals (nnzb, bs, bs), all contiguous on one device."""
+        import torch
+        if self.rowptr.dtype != torch.int32 or tuple(self.rowptr.shape) != (self.mb + 1,):
+            raise ValueError(f"{name}: rowptr must be int32 of shape ({self.mb + 1},)")
+        if self.colidx.dtype != torch.int32 or self.colidx.dim() != 1:
+            raise ValueError(f"{name}: colidx must be a 1-D int32 tensor")
+        if self.nnzb and tuple(self.vals.shape) != (self.nnzb, self.bs, self.bs):
+            raise ValueError(f"{name}: vals must have shape ({self.nnzb}, {self.bs}, {self.bs})")
+        for t in (self.rowptr, self.colidx, self.vals):
+            if not t.is_contiguous():
+                raise ValueError(f"{name}: rowptr, colidx and vals must be contiguous")
+            if t.device != self.rowptr.device:
+                raise ValueError(f"{name}: rowptr, colidx and vals must be on one device")
+
     def _c(self):
         b = _abi.Bsr()
         b.mb, b.nb, b.bs, b.nnzb = self.mb, self.nb, self.bs, self.nnzb
@@ -279,11 +336,22 @@ class BsrMatrix:
         return b
 
 
+def _check_pair(A: BsrMatrix, B: BsrMatrix):
+    A.check("A")
+    B.check("B")
+    if A.nb != B.mb or A.bs != B.bs:
+        raise ValueError(f"A ({A.mb}x{A.nb} blocks of {A.bs}) and B ({B.mb}x{B.nb} blocks of "
+                         f"{B.bs}) do not multiply")
+    if A.rowptr.device != B.rowptr.device:
+        raise ValueError("A and B must be on one device")
+
+
 def spgemm_symbolic(A: BsrMatrix, B: BsrMatrix, stream=None):
     """Row pointer of C = A * B (exclusive scan of per-row block counts) and nnzb(C).
     Synchronises the stream to read nnzb(C) back, as the reference does
     (src/sparse/spgemm/2d/block_spgemm_2d_half_mma.cu:808)."""
     import torch
+    _check_pair(A, B)
     rowptr = torch.empty(A.mb + 1, dtype=torch.int32, device=A.rowptr.device)
     nnz = ctypes.c_int64()
     a, b = A._c(), B._c()
@@ -296,6 +364,10 @@ def spgemm_symbolic(A: BsrMatrix, B: BsrMatrix, stream=None):
 def spgemm_fill(A: BsrMatrix, B: BsrMatrix, rowptrC, nnzbC, stream=None):
     """Ascending block columns of every row of C."""
     import torch
+    _check_pair(A, B)
+    if rowptrC.dtype != torch.int32 or tuple(rowptrC.shape) != (A.mb + 1,) or \
+            not rowptrC.is_contiguous() or rowptrC.device != A.rowptr.device:
+        raise ValueError(f"rowptrC must be a contiguous int32 tensor of shape ({A.mb + 1},)")
     colidx = torch.empty(max(nnzbC, 1), dtype=torch.int32, device=A.rowptr.device)
     a, b = A._c(), B._c()
     check(lib().kami_bsr_spgemm_fill(ctypes.byref(a), ctypes.byref(b),
@@ -311,7 +383,19 @@ def spgemm_numeric(A: BsrMatrix, B: BsrMatrix, rowptrC, colidxC, *, precision=No
     float32 for FP16/BF16 inputs, float64 for FP64.  iters > 1 repeats the block products
     inside the kernel (kami_bsr_spgemm_numeric_repeat)."""
     import torch
+    _check_pair(A, B)
     prec = _precision_of(A.vals, precision)
+    if prec not in (PREC_F64, PREC_F16_F32, PREC_BF16_F32):
+        raise ValueError(f"spgemm_numeric: precision {prec} is not supported")
+    in_dt = _in_dtype(prec)
+    if (A.nnzb and A.vals.dtype != in_dt) or (B.nnzb and B.vals.dtype != in_dt):
+        raise ValueError(f"A and B values must be {in_dt}")
+    if rowptrC.dtype != torch.int32 or tuple(rowptrC.shape) != (A.mb + 1,) or \
+            not rowptrC.is_contiguous() or rowptrC.device != A.rowptr.device:
+        raise ValueError(f"rowptrC must be a contiguous int32 tensor of shape ({A.mb + 1},)")
+    if colidxC.dtype != torch.int32 or colidxC.dim() != 1 or not colidxC.is_contiguous() or \
+            colidxC.device != A.rowptr.device:
+        raise ValueError("colidxC must be a contiguous 1-D int32 tensor")
     out_dt = torch.float64 if prec == PREC_F64 else torch.float32
     nnz = int(colidxC.numel())
     vals = torch.empty((max(nnz, 1), A.bs, A.bs), dtype=out_dt, device=A.rowptr.device)
@@ -348,12 +432,27 @@ def bsr_spmm(A: BsrMatrix, B, C=None, *, b_layout="nk", precision=None, alpha=1.
     reference's layout); "kn": B is (K, n).  K = A.nb * A.bs.  C is (A.mb * A.bs, n),
     float32 for FP16/BF16 inputs, float64 for FP64."""
     import torch
+    A.check("A")
     prec = _precision_of(A.vals if A.nnzb else B, precision)
+    if prec not in (PREC_F64, PREC_F16_F32, PREC_BF16_F32):
+        raise ValueError(f"bsr_spmm: precision {prec} is not supported")
     kn = b_layout == "kn"
+    if B.dim() != 2 or B.stride(1) != 1:
+        raise ValueError("bsr_spmm expects a 2-D B with unit column stride")
     n = int(B.shape[1] if kn else B.shape[0])
+    kdim = A.nb * A.bs
+    if int(B.shape[0] if kn else B.shape[1]) != kdim:
+        raise ValueError(f"B must have {kdim} {'rows' if kn else 'columns'} (A.nb * A.bs)")
+    in_dt = _in_dtype(prec)
+    if B.dtype != in_dt or (A.nnzb and A.vals.dtype != in_dt):
+        raise ValueError(f"A and B values must be {in_dt}")
     out_dt = torch.float64 if prec == PREC_F64 else torch.float32
     if C is None:
         C = torch.zeros((A.mb * A.bs, n), dtype=out_dt, device=B.device)
+    if C.dim() != 2 or tuple(C.shape) != (A.mb * A.bs, n) or C.dtype != out_dt or C.stride(1) != 1:
+        raise ValueError(f"C must be {out_dt} of shape {(A.mb * A.bs, n)} with unit column stride")
+    if C.device != B.device or A.rowptr.device != B.device:
+        raise ValueError("A, B and C must be on one device")
     a = A._c()
     check(lib().kami_bsr_spmm(ctypes.byref(a), ctypes.c_void_p(B.data_ptr()), n, B.stride(0),
                               B_KN if kn else B_NK, ctypes.c_void_p(C.data_ptr()), C.stride(0),

@@ -374,6 +374,8 @@ static kami_status prepare_args(const kami_gemm_desc* d, const void* A, const vo
     args.sb = stride_b ? stride_b : (d->ldb ? b_span : (int64_t)d->n * d->k);
     args.sc = stride_c ? stride_c : (d->ldc ? c_span : (int64_t)d->m * d->n);
     if (count > 1 && ((args.sa * ei) % 16 || (args.sb * ei) % 16)) return KAMI_EINVAL;
+    // problems whose C regions overlap would race on the shared elements
+    if (count > 1 && args.sc < c_span) return KAMI_EINVAL;
     args.alpha = d->alpha;
     args.beta = d->beta;
     args.iters = d->iters;
@@ -387,13 +389,12 @@ kami_status kami_block_gemm(const kami_gemm_desc* d, const void* A, const void* 
     BlockArgs args;
     kami_status st = prepare_args(d, A, B, C, count, stride_a, stride_b, stride_c, args);
     if (st != KAMI_OK || count == 0) return st;
-    if (is_split(d->precision)) {
-        st = device_check();
-        if (st != KAMI_OK) return st;
-        return split_launch(d, args, count, (hipStream_t)stream);
-    }
+    if (is_split(d->precision)) return split_launch(d, args, count, (hipStream_t)stream);
     const BlockInstance* in = resolve_block(d, &st);
     if (!in) return st;
+    // one workgroup per problem: the grid's work-items must stay below 2^32
+    if (in->pipe != 2 && (uint64_t)count * (uint64_t)in->threads >= (1ull << 32))
+        return KAMI_ENOTSUP;
     st = device_check();
     if (st != KAMI_OK) return st;
     hipError_t e = in->launch(args, count, (hipStream_t)stream);

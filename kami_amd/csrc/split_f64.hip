@@ -850,6 +850,11 @@ kami_status split_launch(const kami_gemm_desc* d, const BlockArgs& b, int64_t co
     kami_status st = split_query(d, &cfg);
     if (st != KAMI_OK) return st;
     const SplitInstance* in = resolve_split(d);
+    // one workgroup per problem (pipe 1, resident): work-items must stay below 2^32
+    if (in->pipe != 2 && (uint64_t)count * (uint64_t)in->threads >= (1ull << 32))
+        return KAMI_ENOTSUP;
+    st = device_check();
+    if (st != KAMI_OK) return st;
     SplitArgs a;
     a.A = reinterpret_cast<const double*>(b.A);
     a.B = reinterpret_cast<const double*>(b.B);

@@ -124,10 +124,11 @@ kami_status kami_split_slices(const double* X, int64_t rows, int64_t k, int64_t 
     if (ldx == 0) ldx = k;
     if (ldx < k) return KAMI_EINVAL;
     if (rows == 0 || k == 0) return KAMI_OK;
+    // columns span one grid dimension of 256-thread workgroups, below 2^32 work-items
+    if ((k + 255) / 256 * 256 >= ((int64_t)1 << 32)) return KAMI_ENOTSUP;
     kami_status st = device_check();
     if (st != KAMI_OK) return st;
     SliceArgs a{X, digits, exps, rows, k, ldx, rows * k, (int32_t)((k + 31) / 32), slices};
-    if (k > ((int64_t)1 << 38)) return KAMI_ENOTSUP;  // columns span one grid dimension
     const dim3 grid((unsigned)((k + 255) / 256), (unsigned)std::min<int64_t>(rows, 65535));
     hipStream_t s = (hipStream_t)stream;
     if (precision == KAMI_PREC_F64_SPLIT_F16)

@@ -256,3 +256,23 @@ def test_split_slices_and_generate_validation_without_device():
     assert gen(count=0) == _abi.OK
     if lib.kami_device_check() == _abi.ENODEV:
         assert gen() == _abi.ENODEV
+
+
+def test_launch_limits_and_overlapping_outputs_without_device():
+    """Requests whose grid would reach 2^32 work-items come back ENOTSUP, and problems
+    whose C regions overlap (0 < stride_c < span of one C) EINVAL, before any device
+    call."""
+    lib = K.lib()
+    big = 0x7fffffff
+    d = K.make_desc(64, 64, 64, K.PREC_F64, partition="2d", pipeline=1)
+    cfg = K.query(64, 64, 64, K.PREC_F64, partition="2d", pipeline=1)
+    assert cfg["pipeline"] == 1 and big * cfg["threads"] >= 1 << 32
+    p = ctypes.c_void_p(1 << 20)
+    assert lib.kami_block_gemm(ctypes.byref(d), p, p, p, big, 0, 0, 0, None) == _abi.ENOTSUP
+    ds = K.make_desc(128, 128, 128, K.PREC_F64_SPLIT_F16, pipeline=1)
+    assert lib.kami_block_gemm(ctypes.byref(ds), p, p, p, big, 0, 0, 0, None) == _abi.ENOTSUP
+    assert lib.kami_block_gemm(ctypes.byref(d), p, p, p, 2, 0, 0, 64 * 64 - 16, None) == _abi.EINVAL
+    digits = ctypes.c_void_p(1 << 21)
+    exps = ctypes.c_void_p(1 << 22)
+    assert lib.kami_split_slices(p, 1, 1 << 32, 0, K.PREC_F64_SPLIT_F16, 6, digits, exps,
+                                 None) == _abi.ENOTSUP
