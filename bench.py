@@ -3,9 +3,12 @@
 
 Default workload: FP64 block GEMM C = A B^T at M = N = K = 128 over 4096 independent
 problems (one per workgroup), entries sign-random with log10|x| ~ U[-4, 4].  It is timed
-two ways on the same inputs: native FP64 MFMA and FP64 through FP16 slices on MFMA
-(split precision).  `value` is the TFLOPS (2MNK per problem, FP64-equivalent) of the
-path named by --path (default: the faster of the two that meets the 1e-13 error bound).
+on the same inputs as native FP64 MFMA and as split-precision FP64: int8 digits on int8
+MFMA (split_i8), FP16 digits on FP16 MFMA (split: per-chunk exponents; split_fullk:
+exponents over the whole K).  `value` is the TFLOPS (2MNK per problem, FP64-equivalent)
+of the path named by --path (default: the fastest path whose error against the
+compensated Dot2 oracle is at most 1e-13 componentwise, |C - C*| <= 1e-13 (|A||B|), on a
+sample of problems).
 
 Inputs are resident in HBM before timing.  W untimed warm-up steps, then K timed steps
 bracketed by device synchronisation; one step is one library call over the whole batch.
@@ -15,7 +18,7 @@ start run ~15 % slower than the first few) and then settles, and the settled rat
 what a long-running caller sees.
 The roofline object describes the dominant kernel of the reported path; the CPU
 baseline times the oracle's OpenMP FP64 triple loop (the reference's CPU check loop)
-on the host cores.
+on the host cores, and `cpu_baselines` the reference's CPU loops of the other workloads.
 """
 from __future__ import annotations
 
@@ -93,6 +96,73 @@ def cpu_model():
     return None
 
 
+RECORD_KEYS = {"native": "w3_native", "split": "w3_split", "split_i8": "w3_split_i8"}
+
+
+def _timed_passes(fn, budget_s):
+    fn()  # warm-up
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        reps += 1
+        dt = time.perf_counter() - t0
+        if dt >= budget_s or reps >= 200:
+            return reps, dt
+
+
+def cpu_baselines_other(budget_s=2.5):
+    """The reference's CPU loops for the other workloads, each timed for about `budget_s`
+    seconds on the host cores (OpenMP over problems / block rows): FP64 64^3 x 1024 (the
+    triple loop, F64-1D:294-304), FP16 128^3 x 8192 (the FP16 check loop with double
+    accumulation, F16-1D:285-296), the 20000 uneven FP64 problems (BAT-1D:304-318 over
+    problems of their own sizes) and the 8192^2 block SpGEMM with the SPA (symbolic with
+    the column fill, and numeric, timed separately, SPG-1D:285-370)."""
+    import oracle
+    from kami_amd import workloads as W
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    out = {"cores": cores, "kind": "port"}
+    rng = np.random.default_rng(SEED + 98)
+    A = rng.uniform(-1, 1, (1024, 64, 64))
+    B = rng.uniform(-1, 1, (1024, 64, 64))
+    reps, dt = _timed_passes(lambda: oracle.gemm_f64(A, B, nthreads=cores), budget_s)
+    out["w1_f64_64cubed_x1024"] = {"tflops": 2.0 * 64 ** 3 * 1024 * reps / dt / 1e12,
+                                   "sample": f"{reps} passes over all 1024 problems, {dt:.1f} s"}
+    cnt = 8192
+    Ah = rng.standard_normal((cnt, 128, 128)).astype(np.float16).view(np.uint16)
+    Bh = rng.standard_normal((cnt, 128, 128)).astype(np.float16).view(np.uint16)
+    reps, dt = _timed_passes(lambda: oracle.gemm_u16(Ah, Bh, nthreads=cores), budget_s)
+    out["w2_f16_128cubed_x8192"] = {"tflops": 2.0 * 128 ** 3 * cnt * reps / dt / 1e12,
+                                    "sample": f"{reps} passes over all {cnt} problems, {dt:.1f} s"}
+    del Ah, Bh
+    sizes = W.uneven_sizes(np.random.default_rng(SEED + 3), 20000)
+    na = int((sizes[:, 0] * sizes[:, 2]).sum())
+    nb = int((sizes[:, 1] * sizes[:, 2]).sum())
+    Au, Bu = rng.uniform(-1, 1, na), rng.uniform(-1, 1, nb)
+    reps, dt = _timed_passes(lambda: oracle.batched_uneven_f64(sizes, Au, Bu, nthreads=cores),
+                             budget_s)
+    flop = 2.0 * float((sizes[:, 0] * sizes[:, 1] * sizes[:, 2]).sum())
+    out["w4_batched_uneven_x20000_f64"] = {"tflops": flop * reps / dt / 1e12,
+                                           "sample": f"{reps} passes over all 20000 problems, {dt:.1f} s"}
+    rp, ci, vals = W.spgemm_input(np.random.default_rng(SEED + 4))
+    mb, bs = 256, 32
+    v16 = vals.view(np.uint16)
+
+    def symbolic():
+        rpc, _ = oracle.spgemm_symbolic(mb, mb, rp, ci, rp, ci)
+        oracle.spgemm_fill(mb, mb, rp, ci, rp, ci, rpc)
+    reps, dt = _timed_passes(symbolic, budget_s / 2)
+    rpc, nnzc = oracle.spgemm_symbolic(mb, mb, rp, ci, rp, ci)
+    sym_ms = dt / reps * 1e3
+    reps, dt = _timed_passes(lambda: oracle.spgemm_numeric(mb, mb, bs, rp, ci, v16, rp, ci, v16, rpc),
+                             budget_s / 2)
+    nnzcub = oracle.spgemm_nnzcub(ci, rp)
+    out["w5_block_spgemm"] = {"symbolic_ms": sym_ms, "numeric_ms": dt / reps * 1e3,
+                              "numeric_tflops": 2.0 * nnzcub * bs ** 3 * reps / dt / 1e12,
+                              "nnzb_C": int(nnzc)}
+    return out
+
+
 def measured_traffic(path, cfg, m, count):
     """HBM bytes per launch of the path's kernel from the PMC record in profiles/traffic.json
     (FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy), when that record was taken
@@ -100,7 +170,7 @@ def measured_traffic(path, cfg, m, count):
     rec = None
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-            rec = json.load(f).get({"native": "w3_native", "split": "w3_split"}.get(path, ""))
+            rec = json.load(f).get(RECORD_KEYS.get(path, ""))
     except (OSError, ValueError):
         return None
     if not rec or m != 128 or count != 4096:
@@ -116,7 +186,7 @@ def measured_lds(path, cfg, m, count):
     kernel has one width, else the instruction count alone; None without a record."""
     try:
         with open(os.path.join(ROOT, "profiles", "lds.json")) as f:
-            rec = json.load(f).get({"native": "w3_native", "split": "w3_split"}.get(path, ""))
+            rec = json.load(f).get(RECORD_KEYS.get(path, ""))
     except (OSError, ValueError):
         return None
     if not rec or m != 128 or count != 4096:
@@ -146,7 +216,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--path", choices=["auto", "native", "split", "split_fullk"], default="auto")
+    ap.add_argument("--path", choices=["auto", "native", "split_i8", "split", "split_fullk"],
+                    default="auto")
     ap.add_argument("--m", type=int, default=128)
     ap.add_argument("--problems", type=int, default=4096)
     ap.add_argument("--err-sample", type=int, default=64)
@@ -178,10 +249,11 @@ def main():
 
     results = {}
     outs = {}
-    # split: persistent form with per-chunk exponents (the default); split_fullk: one
-    # workgroup per problem with exponents over the full K, reported for comparison
-    paths = [("native", K.PREC_F64, 0), ("split", K.PREC_F64_SPLIT_F16, 0),
-             ("split_fullk", K.PREC_F64_SPLIT_F16, 1)]
+    # split_i8: int8 digits, persistent, exponents over the whole K; split: FP16 digits,
+    # persistent with per-chunk exponents; split_fullk: FP16 digits, one workgroup per
+    # problem with exponents over the whole K
+    paths = [("native", K.PREC_F64, 0), ("split_i8", K.PREC_F64_SPLIT_I8, 0),
+             ("split", K.PREC_F64_SPLIT_F16, 0), ("split_fullk", K.PREC_F64_SPLIT_F16, 1)]
     for name, prec, pipe in paths:
         try:
             cfg = K.query(m, n, k, prec, pipeline=pipe)
@@ -203,8 +275,9 @@ def main():
         print(f"[bench] {name}: {ms:.4f} ms/step  {tflops:.2f} TFLOPS  err={err}",
               file=sys.stderr, flush=True)
 
-    ok = [p for p in ("split", "split_fullk", "native") if results.get(p, {}).get("available")
-          and (p == "native" or results[p]["max_rel_err"]["normwise"] <= 1e-13)]
+    ok = [p for p in ("split_i8", "split", "split_fullk", "native")
+          if results.get(p, {}).get("available")
+          and (p == "native" or results[p]["max_rel_err"]["componentwise"] <= 1e-13)]
     if args.path == "auto":
         chosen = max(ok, key=lambda p: results[p]["tflops"])
     else:
@@ -225,6 +298,8 @@ def main():
         roof = {"bound": "hbm", "achieved": gbs, "peak": PEAK["hbm_gbs"], "unit": "GB/s",
                 "frac": gbs / PEAK["hbm_gbs"], "traffic": traffic}
     cpu = None if args.no_cpu else cpu_baseline_f64(m, n, k, count)
+    dtypes = {"native": "f64", "split_i8": "i8-digits/i32-acc -> f64",
+              "split": "f16-digits/f32-acc -> f64", "split_fullk": "f16-digits/f32-acc -> f64"}
     line = {
         "metric": METRIC,
         "value": r["tflops"],
@@ -236,7 +311,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if chosen == "native" else "f16-slices/f32-acc -> f64",
+        "dtype": dtypes[chosen],
         "data": "synthetic",
         "config": {"workload": f"FP64 block GEMM {m}x{n}x{k}, {count} problems, log-uniform "
                                f"|x| in [1e-4, 1e4] with random signs; path={chosen}",
@@ -264,7 +339,43 @@ def main():
             print(f"[bench] {name} done in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
                   flush=True)
         line["workloads"] = others
+        if not args.no_cpu:
+            try:
+                line["cpu_baselines"] = cpu_baselines_other()
+            except Exception as e:  # reported, not hidden
+                line["cpu_baselines"] = {"error": f"{type(e).__name__}: {e}"}
+    line["targets"] = targets(results, line.get("workloads", {}))
     print(json.dumps(line))
+
+
+def targets(results, workloads):
+    """The two stated targets in one small object (last in the line): split-precision
+    FP64 against native FP64 at the same shapes, streaming (w3, 128^3 x 4096) and
+    resident-repeat (w3r, 64^3 x 4096), with each split path's componentwise error; and
+    the best FP16 resident-repeat fraction of the FP16 MFMA peak per w2 shape."""
+    t = {"bound_componentwise": 1e-13}
+    nat = results.get("native", {})
+    for p in ("split_i8", "split"):
+        r = results.get(p, {})
+        if r.get("available") and nat.get("available"):
+            t[f"w3_{p}_over_native"] = r["tflops"] / nat["tflops"]
+            t[f"w3_{p}_componentwise"] = r["max_rel_err"]["componentwise"]
+    w3r = workloads.get("w3r_resident_f64_64cubed_x4096", {})
+    for p in ("split_i8", "split_f16"):
+        if f"{p}_over_native" in w3r:
+            t[f"w3r_{p}_over_native"] = w3r[f"{p}_over_native"]
+            t[f"w3r_{p}_componentwise"] = w3r[p]["max_rel_err_vs_torch_fp64"]["componentwise"]
+    w2 = workloads.get("w2_f16_block_gemm", {})
+    best = {}
+    for shape, parts in w2.items() if isinstance(w2, dict) else []:
+        fr = [(v["resident_frac_f16_peak"], k) for k, v in parts.items() if v.get("available", True)
+              and "resident_frac_f16_peak" in v]
+        if fr:
+            f, part = max(fr)
+            best[shape] = [round(f, 3), part]
+    if best:
+        t["w2_best_resident_frac_f16_peak"] = best
+    return t
 
 
 if __name__ == "__main__":

@@ -58,7 +58,9 @@ typedef enum kami_precision {
     KAMI_PREC_F16_F32 = 1,        /* A,B IEEE binary16; C float; FP32 accumulation */
     KAMI_PREC_BF16_F32 = 2,       /* A,B bfloat16; C float; FP32 accumulation */
     KAMI_PREC_F64_SPLIT_F16 = 3,  /* A,B,C double; FP16 slices on MFMA, FP64 recombination */
-    KAMI_PREC_F64_SPLIT_BF16 = 4  /* A,B,C double; BF16 slices on MFMA, FP64 recombination */
+    KAMI_PREC_F64_SPLIT_BF16 = 4, /* A,B,C double; BF16 slices on MFMA, FP64 recombination */
+    KAMI_PREC_F64_SPLIT_I8 = 5    /* A,B,C double; int8 digits of 62-bit fixed point on int8
+                                     MFMA with exact int32 sums, FP64 recombination */
 } kami_precision;
 
 typedef enum kami_partition {

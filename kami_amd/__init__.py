@@ -12,11 +12,12 @@ import os
 from . import _abi
 from ._abi import (B_KN, B_NK, GEN_RAMP, GEN_RAND_R3, PART_1D, PART_2D, PART_3D, PART_AUTO,
                    PREC_BF16_F32, PREC_F16_F32, PREC_F64, PREC_F64_SPLIT_BF16,
-                   PREC_F64_SPLIT_F16, KamiError, check, lib)
+                   PREC_F64_SPLIT_F16, PREC_F64_SPLIT_I8, KamiError, check, lib)
 
 __all__ = [
     "block_gemm", "query", "make_desc", "KamiError", "lib", "version", "device_ok",
     "PREC_F64", "PREC_F16_F32", "PREC_BF16_F32", "PREC_F64_SPLIT_F16", "PREC_F64_SPLIT_BF16",
+    "PREC_F64_SPLIT_I8",
     "PART_AUTO", "PART_1D", "PART_2D", "PART_3D", "B_NK", "B_KN", "GEN_RAMP", "GEN_RAND_R3",
 ]
 
@@ -84,7 +85,8 @@ def _in_dtype(prec):
     import torch
     if not _IN_DTYPE:
         _IN_DTYPE.update({PREC_F64: torch.float64, PREC_F64_SPLIT_F16: torch.float64,
-                          PREC_F64_SPLIT_BF16: torch.float64, PREC_F16_F32: torch.float16,
+                          PREC_F64_SPLIT_BF16: torch.float64, PREC_F64_SPLIT_I8: torch.float64,
+                          PREC_F16_F32: torch.float16,
                           PREC_BF16_F32: torch.bfloat16})
     if prec not in _IN_DTYPE:
         raise ValueError(f"unknown precision {prec}")

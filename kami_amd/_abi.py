@@ -23,7 +23,8 @@ STATUS_NAMES = {OK: "KAMI_OK", EINVAL: "KAMI_EINVAL", ENOTSUP: "KAMI_ENOTSUP",
                 ENODEV: "KAMI_ENODEV"}
 
 # kami_precision
-PREC_F64, PREC_F16_F32, PREC_BF16_F32, PREC_F64_SPLIT_F16, PREC_F64_SPLIT_BF16 = range(5)
+(PREC_F64, PREC_F16_F32, PREC_BF16_F32, PREC_F64_SPLIT_F16, PREC_F64_SPLIT_BF16,
+ PREC_F64_SPLIT_I8) = range(6)
 # kami_partition
 PART_AUTO, PART_1D, PART_2D, PART_3D = range(4)
 # kami_b_layout

@@ -80,8 +80,7 @@ kami_status device_check() {
 // 16 rows for FP64; FP16 uses the fewest waves whose accumulators fit.
 static int default_ranks(int prec, int part, int m, int n, int k) {
     (void)k;
-    const bool f64 = (prec == KAMI_PREC_F64 || prec == KAMI_PREC_F64_SPLIT_F16 ||
-                      prec == KAMI_PREC_F64_SPLIT_BF16);
+    const bool f64 = prec == KAMI_PREC_F64 || is_split(prec);
     const int mn = std::max(m, n);
     if (part == KAMI_PART_1D) {
         if (f64) return std::max(1, m / 16);
@@ -150,7 +149,7 @@ static bool get_tuned_choice(int prec, int m, int n, int k, TuneVal* out) {
 kami_status validate_desc(const kami_gemm_desc* d) {
     if (!d) return KAMI_EINVAL;
     if (d->m <= 0 || d->n <= 0 || d->k <= 0) return KAMI_EINVAL;
-    if (d->precision < KAMI_PREC_F64 || d->precision > KAMI_PREC_F64_SPLIT_BF16) return KAMI_EINVAL;
+    if (d->precision < KAMI_PREC_F64 || d->precision > KAMI_PREC_F64_SPLIT_I8) return KAMI_EINVAL;
     if (d->partition < KAMI_PART_AUTO || d->partition > KAMI_PART_3D) return KAMI_EINVAL;
     if (d->ranks < 0 || d->iters < 1) return KAMI_EINVAL;
     if (d->b_layout != KAMI_B_NK && d->b_layout != KAMI_B_KN) return KAMI_EINVAL;

@@ -34,7 +34,8 @@ int elt_in(int prec);
 int elt_out(int prec);
 
 inline bool is_split(int prec) {
-    return prec == KAMI_PREC_F64_SPLIT_F16 || prec == KAMI_PREC_F64_SPLIT_BF16;
+    return prec == KAMI_PREC_F64_SPLIT_F16 || prec == KAMI_PREC_F64_SPLIT_BF16 ||
+           prec == KAMI_PREC_F64_SPLIT_I8;
 }
 kami_status split_query(const kami_gemm_desc* d, kami_gemm_config* out);
 kami_status split_launch(const kami_gemm_desc* d, const BlockArgs& args, int64_t count,

@@ -25,7 +25,7 @@
 #include <cmath>
 #include <cstdint>
 
-#include "kami_internal.hpp"
+#include "split_common.hpp"
 
 namespace kami {
 
@@ -104,18 +104,6 @@ struct SplitGeom {
     // the persistent form adds the two smallest levels in FP32 before converting them
     static_assert(G >= 3 && level_bound(BETA, S, G - 1, KC) * pow2(-24 - BETA * (G - 1)) <= pow2(-50),
                   "the two smallest levels are combined in FP32");
-};
-
-struct SplitArgs {
-    const double* A;
-    const double* B;
-    double* C;
-    int64_t sa, sb, sc, lda, ldb, ldc;
-    double alpha, beta;
-    // persistent form only: K chunks per problem and the grid of BM x BN tiles of C per
-    // problem (1 x 1 for a block problem)
-    int32_t nch, tiles_m, tiles_n;
-    int32_t iters;  // resident form: repetitions of the product
 };
 
 __device__ inline int exp_of(double x) {
@@ -764,15 +752,6 @@ hipError_t launch_split_resident(const SplitArgs& a, int64_t count, hipStream_t 
     return hipGetLastError();
 }
 
-typedef hipError_t (*split_launch_fn)(const SplitArgs&, int64_t, hipStream_t);
-
-struct SplitInstance {
-    int prec, m, n, k, slices, pm, pn, threads, lds;
-    split_launch_fn launch;
-    int pipe;      // 1: one workgroup per problem, exponents over full K; 2: persistent
-    int resident;  // 1: resident-repeat form, used for iters > 1 only
-};
-
 template <class SAT, int PREC, int BM, int BN, int BK, int PM, int PN, int BETA, int S, int G>
 static void add_split(std::vector<SplitInstance>& v) {
     using Geo = SplitGeom<SAT, BM, BN, BK, PM, PN, BETA, S, G>;
@@ -798,6 +777,7 @@ static const std::vector<SplitInstance>& split_registry() {
         add_split<AtomF16x32, F16, 128, 128, 128, 4, 2, 9, 6, 7>(r);
         add_split<AtomF16x32, F16, 128, 128, 128, 4, 2, 9, 7, 8>(r);
         add_split<AtomF16x32, F16, 64, 64, 64, 2, 2, 9, 6, 7>(r);
+        add_split<AtomF16x32, F16, 64, 64, 64, 2, 2, 9, 7, 8>(r);
         // BF16 digits: 8 bits; default 7 slices (levels <= 8, 28 products)
         add_split<AtomBF16x32, BF, 128, 128, 128, 4, 2, 8, 7, 8>(r);
         add_split<AtomBF16x32, BF, 64, 64, 64, 2, 2, 8, 7, 8>(r);
@@ -806,41 +786,59 @@ static const std::vector<SplitInstance>& split_registry() {
         add_split_resident<AtomF16x32, F16, 64, 64, 64, 4, 2, 9, 6, 7, 8>(r);
         add_split_resident<AtomBF16x32, BF, 64, 64, 64, 4, 2, 8, 7, 8, 8>(r);
         add_split_resident<AtomF16x32, F16, 32, 32, 32, 2, 2, 9, 6, 7, 8>(r);
+        add_split_resident<AtomF16x32, F16, 64, 64, 64, 4, 2, 9, 7, 8, 8>(r);
+        add_split_resident<AtomF16x32, F16, 32, 32, 32, 2, 2, 9, 7, 8, 8>(r);
         add_split_resident<AtomBF16x32, BF, 32, 32, 32, 2, 2, 8, 7, 8, 8>(r);
+        add_split_i8_instances(r);
         return r;
     }();
     return v;
 }
 
-// pipeline 0 selects the persistent form for iters = 1 and the resident form (one
-// workgroup per problem) for iters > 1.
+// Default digit count per precision: FP16 7 (9-bit digits, 63 bits), BF16 7 (8-bit
+// digits), int8 8 (62-bit fixed point).
+static int default_slices(int prec) {
+    return prec == KAMI_PREC_F64_SPLIT_I8 ? 8 : 7;
+}
+
+// FP16/BF16: pipeline 0 selects the persistent form for iters = 1 and the resident form
+// (one workgroup per problem) for iters > 1.  Int8 has one persistent form for any iters.
 static const SplitInstance* resolve_split(const kami_gemm_desc* d) {
-    const int want = d->slices ? d->slices : (d->precision == KAMI_PREC_F64_SPLIT_F16 ? 6 : 7);
+    const int want = d->slices ? d->slices : default_slices(d->precision);
     const int rep = d->iters > 1 ? 1 : 0;
     const int pipe = d->pipeline ? d->pipeline : (rep ? 1 : 2);
-    for (const auto& in : split_registry())
-        if (in.prec == d->precision && in.m == d->m && in.n == d->n && in.k == d->k &&
-            in.slices == want && in.pipe == pipe && in.resident == rep)
+    for (const auto& in : split_registry()) {
+        if (in.prec != d->precision || in.m != d->m || in.n != d->n || in.k != d->k ||
+            in.slices != want)
+            continue;
+        if (in.resident == 2) {
+            if (d->pipeline == 0 || d->pipeline == in.pipe) return &in;
+        } else if (in.pipe == pipe && in.resident == rep) {
             return &in;
+        }
+    }
     return nullptr;
 }
 
 kami_status split_query(const kami_gemm_desc* d, kami_gemm_config* out) {
     if (d->b_layout != KAMI_B_NK || d->partition == KAMI_PART_3D) return KAMI_ENOTSUP;
+    const bool i8 = d->precision == KAMI_PREC_F64_SPLIT_I8;
+    // int8: each wave owns whole rows of C (1D); FP16/BF16: a 2D wave grid
+    if (d->partition == (i8 ? KAMI_PART_2D : KAMI_PART_1D)) return KAMI_ENOTSUP;
     const SplitInstance* in = resolve_split(d);
     if (!in) return KAMI_ENOTSUP;
-    out->partition = KAMI_PART_2D;
+    out->partition = i8 ? KAMI_PART_1D : KAMI_PART_2D;
     out->ranks = in->pm;
     out->waves_m = in->pm;
     out->waves_n = in->pn;
     out->waves_k = 1;
     out->threads = in->threads;
-    out->k_stage = in->resident ? in->k : 32;
+    out->k_stage = (in->resident || i8) ? in->k : 32;
     out->lds_bytes = in->lds;
     out->slices = in->slices;
     out->instance = (int32_t)(in - split_registry().data());
     out->pipeline = in->pipe;
-    out->prefetch = in->resident ? 0 : 1;
+    out->prefetch = in->resident == 1 ? 0 : 1;
     return KAMI_OK;
 }
 
@@ -853,6 +851,13 @@ kami_status split_launch(const kami_gemm_desc* d, const BlockArgs& b, int64_t co
     // one workgroup per problem (pipe 1, resident): work-items must stay below 2^32
     if (in->pipe != 2 && (uint64_t)count * (uint64_t)in->threads >= (1ull << 32))
         return KAMI_ENOTSUP;
+    if (d->precision == KAMI_PREC_F64_SPLIT_I8) {
+        // the int8 kernel addresses a problem's rows with 32-bit byte offsets
+        const int64_t lim = (int64_t)1 << 28;  // elements
+        if ((int64_t)(d->m - 1) * b.lda + d->k >= lim || (int64_t)(d->n - 1) * b.ldb + d->k >= lim ||
+            (int64_t)(d->m - 1) * b.ldc + d->n >= lim)
+            return KAMI_ENOTSUP;
+    }
     st = device_check();
     if (st != KAMI_OK) return st;
     SplitArgs a;
@@ -880,7 +885,8 @@ kami_status split_launch(const kami_gemm_desc* d, const BlockArgs& b, int64_t co
 // every problem is cut into tiles of C, each a work unit of the persistent kernel.
 kami_status split_gemm_tiled(const kami_gemm_desc* d, const BlockArgs& b, int64_t count,
                              hipStream_t stream) {
-    if (!is_split(d->precision)) return KAMI_ENOTSUP;
+    // the int8 form exists for whole block problems only (exponents over the full K)
+    if (!is_split(d->precision) || d->precision == KAMI_PREC_F64_SPLIT_I8) return KAMI_ENOTSUP;
     if (d->b_layout != KAMI_B_NK || d->iters != 1) return KAMI_ENOTSUP;
     if (d->partition != KAMI_PART_AUTO && d->partition != KAMI_PART_2D) return KAMI_ENOTSUP;
     if (d->k % 32 != 0) return KAMI_ENOTSUP;

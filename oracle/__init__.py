@@ -48,6 +48,7 @@ def lib():
             "oracle_batched_gemm_f64": (None, [_c.c_int64, _c.c_int, _c.c_int, _c.c_int, _f64p,
                                                _c.c_int64, _f64p, _c.c_int64, _f64p, _c.c_int64,
                                                _c.c_int, _c.c_int]),
+            "oracle_batched_uneven_f64": (None, [_c.c_int64, _i32p, _f64p, _f64p, _f64p, _c.c_int]),
             "oracle_batched_gemm_u16": (None, [_c.c_int64, _c.c_int, _c.c_int, _c.c_int, _u16p,
                                                _c.c_int64, _u16p, _c.c_int64, _f64p, _c.c_int64,
                                                _c.c_int, _c.c_int, _c.c_int]),
@@ -95,6 +96,19 @@ def gemm_f64(A, B, b_kn=False, nthreads=0):
     n = B.shape[2] if b_kn else B.shape[1]
     C = np.empty((count, m, n), dtype=np.float64)
     lib().oracle_batched_gemm_f64(count, m, n, k, A, m * k, B, n * k, C, m * n, int(b_kn), nthreads)
+    return C
+
+
+def batched_uneven_f64(sizes, A, B, nthreads=0):
+    """C of every problem p = sizes[p] = (m, n, k): A (m x k), B (n x k) and C (m x n)
+    packed one after another in problem order (the batched workload's layout); the
+    reference's per-problem FP64 loop with OpenMP over the batch
+    (src/batched_gemm/KAMI/1d/batched_gemm_1d_double_mma.cu:304-318)."""
+    mnk = np.ascontiguousarray(sizes, dtype=np.int32).reshape(-1)
+    count = mnk.size // 3
+    nc = int(np.sum(mnk[0::3].astype(np.int64) * mnk[1::3]))
+    C = np.empty(nc, dtype=np.float64)
+    lib().oracle_batched_uneven_f64(count, mnk, _c64(A), _c64(B), C, nthreads)
     return C
 
 

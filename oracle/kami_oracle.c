@@ -115,6 +115,32 @@ void oracle_batched_gemm_f64(int64_t count, int m, int n, int k, const double* A
     (void)nthreads;
 }
 
+/* The same loop over problems of their own sizes (the uneven batched workload): problem p
+ * is mnk[3p] x mnk[3p+1] x mnk[3p+2], its A, B (N x K) and C packed one after another in
+ * problem order.  Dynamic scheduling balances the uneven problems over the threads. */
+void oracle_batched_uneven_f64(int64_t count, const int32_t* mnk, const double* A,
+                               const double* B, double* C, int nthreads) {
+    int64_t* off = (int64_t*)malloc(sizeof(int64_t) * 3 * (size_t)(count + 1));
+    if (!off) return;
+    off[0] = off[1] = off[2] = 0;
+    for (int64_t p = 0; p < count; p++) {
+        const int64_t m = mnk[3 * p], n = mnk[3 * p + 1], k = mnk[3 * p + 2];
+        off[3 * p + 3] = off[3 * p] + m * k;
+        off[3 * p + 4] = off[3 * p + 1] + n * k;
+        off[3 * p + 5] = off[3 * p + 2] + m * n;
+    }
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 8) num_threads(nthreads)
+#endif
+    for (int64_t p = 0; p < count; p++) {
+        const int m = mnk[3 * p], n = mnk[3 * p + 1], k = mnk[3 * p + 2];
+        oracle_gemm_f64(m, n, k, A + off[3 * p], k, B + off[3 * p + 1], k, 0, C + off[3 * p + 2], n);
+    }
+    free(off);
+    (void)nthreads;
+}
+
 /* FP16 / BF16 inputs widened to double and accumulated in double, the reference's
  * FP16 validation loop (src/block_gemm/KAMI/1d/fp16/block_gemm_1d_half_mma.cu:
  * 285-296) with its (double) casts. */

@@ -17,6 +17,8 @@ void oracle_gemm_f16(int m, int n, int k, const uint16_t* A, int64_t lda, const 
                      int64_t ldb, int b_kn, double* C, int64_t ldc);
 void oracle_gemm_bf16(int m, int n, int k, const uint16_t* A, int64_t lda, const uint16_t* B,
                       int64_t ldb, int b_kn, double* C, int64_t ldc);
+void oracle_batched_uneven_f64(int64_t count, const int32_t* mnk, const double* A,
+                               const double* B, double* C, int nthreads);
 void oracle_batched_gemm_u16(int64_t count, int m, int n, int k, const uint16_t* A, int64_t sa,
                              const uint16_t* B, int64_t sb, double* C, int64_t sc, int b_kn,
                              int is_bf16, int nthreads);

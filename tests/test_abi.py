@@ -102,7 +102,7 @@ def test_device_check_without_gpu_reports_nodev():
 
 def test_split_precision_query_and_enotsup():
     cfg = K.query(128, 128, 128, K.PREC_F64_SPLIT_F16)
-    assert cfg["slices"] == 6 and cfg["threads"] == 512 and cfg["lds_bytes"] <= 160 * 1024
+    assert cfg["slices"] == 7 and cfg["threads"] == 512 and cfg["lds_bytes"] <= 160 * 1024
     assert K.query(128, 128, 128, K.PREC_F64_SPLIT_BF16)["slices"] == 7
     for kw in [dict(b_layout="kn"), dict(partition="3d"), dict(iters=2), dict(slices=3)]:
         with pytest.raises(K.KamiError) as e:
@@ -110,7 +110,7 @@ def test_split_precision_query_and_enotsup():
         assert e.value.status == _abi.ENOTSUP
     # 64^3 with iters > 1 resolves to the resident-repeat form (whole K on chip); the
     # persistent form takes iters = 1 only
-    for prec, slices in [(K.PREC_F64_SPLIT_F16, 6), (K.PREC_F64_SPLIT_BF16, 7)]:
+    for prec, slices in [(K.PREC_F64_SPLIT_F16, 7), (K.PREC_F64_SPLIT_BF16, 7)]:
         cfg = K.query(64, 64, 64, prec, iters=3)
         assert (cfg["k_stage"], cfg["pipeline"], cfg["prefetch"], cfg["slices"]) == (64, 1, 0, slices)
         assert cfg["threads"] == 512 and cfg["lds_bytes"] <= 160 * 1024
@@ -123,6 +123,34 @@ def test_split_precision_query_and_enotsup():
         with pytest.raises(K.KamiError) as e:
             K.query(32, 32, 32, prec, iters=1)
         assert e.value.status == _abi.ENOTSUP
+
+
+def test_split_i8_query_and_enotsup():
+    """The int8-digit form: 1D (a wave per 16 rows of C), one persistent form for any
+    iters, eight digits, whole-K exponents; K x N B, 2D / 3D partitions, other digit
+    counts, shapes without an instance and kami_split_gemm are ENOTSUP."""
+    for s, waves in [(128, 8), (64, 4)]:
+        for iters in (1, 4):
+            cfg = K.query(s, s, s, K.PREC_F64_SPLIT_I8, iters=iters)
+            assert (cfg["partition"], cfg["ranks"], cfg["slices"], cfg["pipeline"]) == \
+                (K.PART_1D, waves, 8, 2)
+            assert cfg["threads"] == 64 * waves and cfg["k_stage"] == s
+            assert cfg["lds_bytes"] <= 160 * 1024
+        assert K.query(s, s, s, K.PREC_F64_SPLIT_I8, partition="1d")["ranks"] == waves
+    for kw in [dict(b_layout="kn"), dict(partition="2d"), dict(partition="3d"), dict(slices=7),
+               dict(pipeline=1)]:
+        with pytest.raises(K.KamiError) as e:
+            K.query(128, 128, 128, K.PREC_F64_SPLIT_I8, **kw)
+        assert e.value.status == _abi.ENOTSUP
+    with pytest.raises(K.KamiError) as e:
+        K.query(32, 32, 32, K.PREC_F64_SPLIT_I8)
+    assert e.value.status == _abi.ENOTSUP
+    d = K.make_desc(256, 256, 128, K.PREC_F64_SPLIT_I8)
+    p = ctypes.c_void_p(1 << 20)
+    assert K.lib().kami_split_gemm(ctypes.byref(d), p, p, p, 1, 0, 0, 0, None) == _abi.ENOTSUP
+    # rows addressed with 32-bit byte offsets: a problem spanning 2^31 bytes is refused
+    d = K.make_desc(128, 128, 128, K.PREC_F64_SPLIT_I8, lda=1 << 22)
+    assert K.lib().kami_block_gemm(ctypes.byref(d), p, p, p, 1, 0, 0, 0, None) == _abi.ENOTSUP
 
 
 def test_spgemm_numeric_repeat_rejects_iters_below_one():

@@ -74,5 +74,26 @@ def test_bench_prints_one_json_line_with_the_contract_fields():
     assert d["value"] > 0 and d["unit"] == "TFLOPS" and d["higher_is_better"] is True
     roof = d["roofline"]
     assert roof["peak"] > 0 and roof["frac"] == pytest.approx(roof["achieved"] / roof["peak"])
-    assert d["max_rel_err"]["normwise"] <= 1e-13
+    assert d["max_rel_err"]["componentwise"] <= 1e-13
     assert "workload" in d["config"]
+    assert list(d)[-1] == "targets" and d["targets"]["bound_componentwise"] == 1e-13
+
+
+def test_targets_summarise_the_paths_and_workloads():
+    b = _bench_module()
+    res = {"native": {"available": True, "tflops": 40.0},
+           "split_i8": {"available": True, "tflops": 60.0,
+                        "max_rel_err": {"componentwise": 3e-15, "normwise": 1e-16}},
+           "split": {"available": False}}
+    w = {"w3r_resident_f64_64cubed_x4096": {
+             "split_i8_over_native": 1.5, "split_f16_over_native": 1.1,
+             "split_i8": {"max_rel_err_vs_torch_fp64": {"componentwise": 2e-15}},
+             "split_f16": {"max_rel_err_vs_torch_fp64": {"componentwise": 5e-14}}},
+         "w2_f16_block_gemm": {"64x64x64": {"1d": {"resident_frac_f16_peak": 0.5},
+                                            "2d": {"resident_frac_f16_peak": 0.74},
+                                            "3d": {"available": False}}}}
+    t = b.targets(res, w)
+    assert t["w3_split_i8_over_native"] == pytest.approx(1.5)
+    assert "w3_split_over_native" not in t
+    assert t["w3r_split_i8_over_native"] == 1.5 and t["w3r_split_f16_componentwise"] == 5e-14
+    assert t["w2_best_resident_frac_f16_peak"] == {"64x64x64": [0.74, "2d"]}

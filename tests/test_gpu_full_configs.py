@@ -64,8 +64,13 @@ def log_uniform(rng, shape):
     return np.where(rng.random(shape) < 0.5, -1.0, 1.0) * 10.0 ** rng.uniform(-4, 4, shape)
 
 
-@pytest.mark.parametrize("prec", [K.PREC_F64_SPLIT_F16, K.PREC_F64_SPLIT_BF16], ids=["f16", "bf16"])
+@pytest.mark.parametrize("prec", [K.PREC_F64_SPLIT_I8, K.PREC_F64_SPLIT_F16, K.PREC_F64_SPLIT_BF16],
+                         ids=["i8", "f16", "bf16"])
 def test_split_128cubed_x4096_agrees_with_native_on_every_problem(prec, dev):
+    """The benchmark's split workload at full size (FP64 128^3 x 4096, log-uniform).
+    Int8 and FP16 digits (their default 8 and 7 digits) meet 1e-13 componentwise,
+    |C - C*| <= 1e-13 (|A||B|): against native FP64 on every problem and against the Dot2
+    oracle on a sample.  BF16 (7 digits of 8 bits) meets 1e-13 normwise only."""
     rng = np.random.default_rng(0x4B414D49 + 2)
     count, s = 4096, 128
     A = log_uniform(rng, (count, s, s))
@@ -73,14 +78,23 @@ def test_split_128cubed_x4096_agrees_with_native_on_every_problem(prec, dev):
     Ad, Bd = torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev)
     Cn = K.block_gemm(Ad, Bd)
     Cs = K.block_gemm(Ad, Bd, precision=prec)
-    diff = (Cs - Cn).abs().amax(dim=(1, 2)) / Cn.abs().amax(dim=(1, 2))
+    componentwise = prec != K.PREC_F64_SPLIT_BF16
+    if componentwise:
+        absAB = torch.bmm(Ad.abs(), Bd.abs().transpose(1, 2).contiguous())
+        diff = ((Cs - Cn).abs() / absAB).amax(dim=(1, 2))
+    else:
+        diff = (Cs - Cn).abs().amax(dim=(1, 2)) / Cn.abs().amax(dim=(1, 2))
+    print(f"  max difference to native FP64: {float(diff.max()):.3e}", flush=True)
     assert float(diff.max()) <= 1e-13
     idx = np.arange(0, count, count // 16)
-    exact, _ = oracle.gemm_dot2(A[idx], B[idx])
+    exact, absC = oracle.gemm_dot2(A[idx], B[idx])
     for C in (Cs, Cn):
         got = C[torch.from_numpy(idx).to(dev)].cpu().numpy()
-        err = np.max(np.abs(got - exact), axis=(1, 2)) / np.max(np.abs(exact), axis=(1, 2))
-        assert float(err.max()) <= 1e-13
+        if componentwise:
+            err = np.max(np.abs(got - exact) / absC)
+        else:
+            err = np.max(np.max(np.abs(got - exact), axis=(1, 2)) / np.max(np.abs(exact), axis=(1, 2)))
+        assert float(err) <= 1e-13
 
 
 def uneven_sizes(rng, count, lo=16, hi=256):

@@ -1,7 +1,8 @@
 """GPU parity of split-precision FP64 (FP16 / BF16 digit slices on MFMA).
 
 The oracle is the compensated Dot2 product (twice-working-precision accurate).  The
-error bound stated for the split path is normwise per problem:
+default FP16 form (seven 9-bit digits) meets 1e-13 componentwise on the log-uniform
+workload, |C - C*| <= 1e-13 (|A||B|); every form meets it normwise per problem,
     max_ij |C - C*| / max_ij |C*|  <= 1e-13,
 and integer-valued inputs (the reference's ramp and rand_r % 3 known answers) must be
 reproduced exactly, since every digit, digit product and level sum is then exact.
@@ -89,9 +90,13 @@ def test_log_uniform_normwise_1e13(prec, slices, pipe, dev):
         C = run(A, B, prec, slices, dev, pipe=pipe)
         exact, absC = oracle.gemm_dot2(A, B)
         err = normwise(C, exact)
-        print(f"  {s}^3 normwise {err:.3e}  componentwise "
-              f"{np.max(np.abs(C - exact) / absC):.3e}", flush=True)
+        comp = float(np.max(np.abs(C - exact) / absC))
+        print(f"  {s}^3 normwise {err:.3e}  componentwise {comp:.3e}", flush=True)
         assert err <= 1e-13
+        if (prec, slices) == (K.PREC_F64_SPLIT_F16, 7):
+            # the default FP16 digit count meets the bound componentwise, |C - C*| <=
+            # 1e-13 (|A||B|); six digits and BF16's seven meet it normwise only
+            assert comp <= 1e-13
         Cn = run(A, B, K.PREC_F64, 0, dev)
         assert normwise(Cn, exact) <= 1e-14
 
@@ -151,8 +156,8 @@ def test_persistent_walks_many_problems(prec, slices, dev):
 
 # resident-repeat forms (iters > 1): 32^3 and 64^3, the default slice count of each
 # digit type
-RESIDENT = [(K.PREC_F64_SPLIT_F16, 6), (K.PREC_F64_SPLIT_BF16, 7)]
-RIDS = ["f16-s6", "bf16-s7"]
+RESIDENT = [(K.PREC_F64_SPLIT_F16, 6), (K.PREC_F64_SPLIT_F16, 7), (K.PREC_F64_SPLIT_BF16, 7)]
+RIDS = ["f16-s6", "f16-s7", "bf16-s7"]
 
 
 @pytest.mark.parametrize("prec,slices", RESIDENT, ids=RIDS)

@@ -88,3 +88,109 @@ def test_fp32_level_sum_exact_up_to_2_pow_24():
     assert float(np.float32(2 ** 24)) == 2 ** 24
     assert float(np.float32(2 ** 24 - 1)) == 2 ** 24 - 1
     assert float(np.float32(2 ** 24 + 1)) != 2 ** 24 + 1
+
+
+# ---- int8 digits of the split path (split_i8.hip) ----------------------------------
+
+def _ldexp_bits(x, e):
+    return np.frombuffer(np.array([np.ldexp(x, e)], dtype=np.float64).tobytes(), np.uint64)[0]
+
+
+def digit_word_model(x, neg_e):
+    """The kernel's digit_word step for step with 32-bit words: scale by 2^-E, place the
+    significand's leading bit at bit 62 (v_alignbit + bit insert), shift right by
+    min(1023 - exponent, 63), two's complement by the sign, add 0x80 to every byte with
+    carries, flip each byte's top bit."""
+    M32 = 0xFFFFFFFF
+    bits = int(_ldexp_bits(x, neg_e))
+    h, l = bits >> 32, bits & M32
+    ex = (h >> 20) & 0x7FF
+    r = min((1023 - ex) & M32, 63)
+    align = ((h << 32 | l) >> 22) & M32
+    mh = (align & 0x3FFFFFFF) | 0x40000000
+    X = ((mh << 32) | ((l << 10) & M32)) >> r
+    s = 0xFFFFFFFFFFFFFFFF if h >> 31 else 0
+    Y = ((X ^ s) + 0x8080808080808080 + (s & 1)) & 0xFFFFFFFFFFFFFFFF
+    return Y ^ 0x8080808080808080
+
+
+def balanced_digits(word):
+    return [((word >> (8 * b)) & 0xFF) - 256 * (((word >> (8 * b)) & 0xFF) >= 128) for b in range(8)]
+
+
+def row_exponent_model(row):
+    """|x| < 2^E from the largest upper word of |x|, completed with all-ones low bits."""
+    hi = max(int(np.frombuffer(np.float64(v).tobytes(), np.uint64)[0]) >> 32 & 0x7FFFFFFF for v in row)
+    return int(np.frexp(np.frombuffer(np.uint64(hi << 32 | 0xFFFFFFFF).tobytes(), np.float64)[0])[1])
+
+
+def test_int8_digits_reconstruct_the_truncated_fixed_point_value():
+    """For every x of a row with |x| < 2^E the eight balanced byte digits d_b in
+    [-128, 127] satisfy sum_b d_b 2^(8b) = trunc(|x| 2^(62 - E)) sign(x) exactly (Python
+    integers and fractions as the exact reference); the row maximum lands in [2^61, 2^62)."""
+    from fractions import Fraction
+    rng = np.random.default_rng(5)
+    rows = [np.where(rng.random(64) < 0.5, -1.0, 1.0) * 10.0 ** rng.uniform(-4, 4, 64),
+            rng.uniform(-1, 1, 64), np.array([0.0, -0.0, 1.0, -1.0, 2.0 ** -40, 3.0, 0.75, 1e-310]),
+            np.array([5e-324 * k for k in (1, 7, 999, 1000)]),       # a row of subnormals
+            np.array([1.7e308, -1e300, 1.0]), np.array([2.0 ** -1022, 2.0 ** -1060])]
+    for row in rows:
+        E = row_exponent_model(row)
+        assert all(abs(Fraction(float(v))) < Fraction(2) ** E for v in row)
+        big = max(abs(Fraction(float(v))) for v in row)
+        for v in row:
+            d = balanced_digits(digit_word_model(float(v), -E))
+            assert all(-128 <= q <= 127 for q in d)
+            X = sum(q << (8 * b) for b, q in enumerate(d))
+            fx = Fraction(float(v))
+            want = int(abs(fx) * Fraction(2) ** (62 - E))  # truncation toward zero
+            assert X == (want if fx >= 0 else -want), (v, E)
+        if big > 0 and big.numerator.bit_length() < 60:
+            top = int(big * Fraction(2) ** (62 - E))
+            assert 2 ** 61 <= top < 2 ** 62 or row_exponent_model(row) < -1021
+
+
+def v_perm(s0, s1, sel):
+    data = (s0 << 32) | s1
+    out = 0
+    for i in range(4):
+        b = (sel >> (8 * i)) & 0xFF
+        assert b < 8
+        out |= ((data >> (8 * b)) & 0xFF) << (8 * i)
+    return out
+
+
+def test_int8_transpose_selectors_gather_one_byte_plane_per_word():
+    """transpose4's v_perm_b32 selectors: out[b] holds byte b of w0, w1, w2, w3."""
+    rng = np.random.default_rng(6)
+    for _ in range(50):
+        w = [int(x) for x in rng.integers(0, 2 ** 63, 4, dtype=np.uint64)]
+        out = []
+        for half in range(2):
+            a, b, c, d = ((x >> (32 * half)) & 0xFFFFFFFF for x in w)
+            ab_lo, ab_hi = v_perm(b, a, 0x05010400), v_perm(b, a, 0x07030602)
+            cd_lo, cd_hi = v_perm(d, c, 0x05010400), v_perm(d, c, 0x07030602)
+            out += [v_perm(cd_lo, ab_lo, 0x05040100), v_perm(cd_lo, ab_lo, 0x07060302),
+                    v_perm(cd_hi, ab_hi, 0x05040100), v_perm(cd_hi, ab_hi, 0x07060302)]
+        for byte in range(8):
+            assert out[byte] == sum(((w[q] >> (8 * byte)) & 0xFF) << (8 * q) for q in range(4))
+
+
+def test_int8_level_sums_fit_int32_and_the_horner_scale():
+    """Level l sums at most min(l, 14 - l) + 1 <= 8 digit products of magnitude <= 2^14 per
+    k, so over K <= 128 every level is below 2^31 (exact in int32).  The kept levels
+    combine as C = 2^(e + f - 124) sum_l T_l 2^(8 (14 - l)) = 2^(e + f - 12) sum_l T_l 2^(-8 l)."""
+    for K in (64, 128):
+        for l in range(15):
+            terms = sum(1 for p in range(8) for q in range(8) if p + q == l)
+            assert terms <= 8 and terms * K * 128 * 128 < 2 ** 31
+    # Horner scale on a random exact example
+    rng = np.random.default_rng(7)
+    da = rng.integers(-128, 128, (8, 5))
+    db = rng.integers(-128, 128, (8, 5))
+    X = [sum(int(da[p, k]) << (8 * (7 - p)) for p in range(8)) for k in range(5)]
+    Y = [sum(int(db[q, k]) << (8 * (7 - q)) for q in range(8)) for k in range(5)]
+    exact = sum(x * y for x, y in zip(X, Y))
+    T = [sum(int(da[p, k]) * int(db[l - p, k]) for k in range(5) for p in range(8) if 0 <= l - p < 8)
+         for l in range(15)]
+    assert sum(t << (8 * (14 - l)) for l, t in enumerate(T)) == exact
