@@ -1,9 +1,10 @@
 // Block-GEMM instances: BF16 inputs, FP32 accumulation (16x16x32 and 16x16x16 bf16 MFMA).
 // Shapes are the reference's FP16 sweep (scripts/fp16_block_square_GH200.sh:11-25,
 // scripts/fp16_block_lowrank_GH200.sh:20-27): squares {16, 32, 64, 128, 192, 256} and
-// low-rank M = N in {32, 64, 128, 192, 256} with K in {16, 32}.  A 3D partition at
-// M = N >= 192 is not offered: its per-wave FP32 accumulator tile exceeds the register
-// file at every legal workgroup size; low-rank K = 16 has no 3D split of K either.
+// low-rank M = N in {32, 64, 128, 192, 256} with K in {16, 32}.  The 3D partition at
+// M = N >= 192 runs in column passes (add_pass_instance): a full-width per-wave FP32
+// accumulator tile at two depths exceeds the register file at every legal workgroup
+// size; K = 16 splits over two depths with a K step of 8 on the 16x16x16 instruction.
 #include "block_gemm_registry.hpp"
 
 namespace kami {
@@ -11,6 +12,7 @@ namespace kami {
 void register_bf16_instances(std::vector<BlockInstance>& v) {
     using A = AtomBF16x32;
     using H = AtomBF16x16;  // K step 16, for K slices that are not multiples of 32
+    using H8 = AtomBF16x16K8;  // K step 8: the 3D split of K = 16
     constexpr int F = 2;
     // 32^3
     add_instance<A, F, 32, 32, 32, 1, 1, 1>(v, 1, 1);
@@ -85,6 +87,14 @@ void register_bf16_instances(std::vector<BlockInstance>& v) {
     add_instance<A, F, 192, 192, 32, 4, 1, 1>(v, 1, 4);
     add_instance<A, F, 192, 192, 32, 12, 1, 1>(v, 1, 12);
     add_instance<A, F, 192, 192, 32, 4, 4, 1>(v, 2, 4);
+
+    // 3D at 192^3, 256^3 and 256^2 x {16, 32} (2 x 2 x 2 waves) in column passes: the
+    // full-width wave tiles (96 x 96 and 128 x 128 FP32 per wave at two depths) exceed
+    // the register file, tiles of BN / SP columns do not
+    add_pass_instance<A, F, 192, 192, 192, 2, 2, 2, 2>(v, 2);
+    add_pass_instance<A, F, 256, 256, 256, 2, 2, 2, 4>(v, 2);
+    add_pass_instance<H, F, 256, 256, 32, 2, 2, 2, 2>(v, 2);
+    add_pass_instance<H8, F, 256, 256, 16, 2, 2, 2, 2>(v, 2);
 
     add_pipe_instance<A, F, 64, 64, 64, 2, 2, 4>(v, 2);
     add_pipe_instance<A, F, 64, 64, 64, 4, 4, 4>(v, 4);

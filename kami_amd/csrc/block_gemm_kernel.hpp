@@ -82,7 +82,11 @@ struct BlockArgs {
 // KFENCE > 0: a scheduling fence after every KFENCE K steps of a stage, so the
 // compiler does not hoist the B fragment loads of a whole long stage ahead of the MFMAs
 // (used by the resident forms, whose single stage covers all of K).
-template <class AT, class G, int KFENCE = 0>
+//
+// SP > 1: the wave tiles of the full BN columns do not fit the register file, so the
+// problem's columns are computed in SP passes of G::BN = BN / SP columns each (B rows and
+// C columns of pass sp start at sp * G::BN); A is read again in every pass.
+template <class AT, class G, int KFENCE = 0, int SP = 1>
 __global__ void __launch_bounds__(G::NT)
 block_gemm_kernel(BlockArgs args) {
     using in_t = typename AT::in_t;
@@ -103,11 +107,15 @@ block_gemm_kernel(BlockArgs args) {
     const int pm = wave / (G::PK * G::PN);
 
     const int64_t prob = blockIdx.x;
-    const in_t* __restrict__ A = reinterpret_cast<const in_t*>(args.A) + prob * args.sa;
-    const in_t* __restrict__ B = reinterpret_cast<const in_t*>(args.B) + prob * args.sb;
-    out_t* __restrict__ C = reinterpret_cast<out_t*>(args.C) + prob * args.sc;
     const int64_t lda = args.lda, ldb = args.ldb, ldc = args.ldc;
     const bool b_kn = args.b_kn != 0;
+    const in_t* __restrict__ A = reinterpret_cast<const in_t*>(args.A) + prob * args.sa;
+#pragma unroll 1
+    for (int sp = 0; sp < SP; ++sp) {
+    if (sp > 0) __syncthreads();  // the previous pass is done with LDS
+    const in_t* __restrict__ B = reinterpret_cast<const in_t*>(args.B) + prob * args.sb +
+                                 (int64_t)sp * G::BN * (b_kn ? 1 : ldb);
+    out_t* __restrict__ C = reinterpret_cast<out_t*>(args.C) + prob * args.sc + sp * G::BN;
 
     accv_t acc[G::IM][G::JN];
 #pragma unroll
@@ -358,12 +366,13 @@ block_gemm_kernel(BlockArgs args) {
                 *dst = (out_t)val;
             }
         }
+    }  // column passes
 }
 
-template <class AT, class G, int KFENCE = 0>
+template <class AT, class G, int KFENCE = 0, int SP = 1>
 hipError_t launch_block_gemm(const BlockArgs& args, int64_t count, hipStream_t stream) {
-    hipLaunchKernelGGL((block_gemm_kernel<AT, G, KFENCE>), dim3((unsigned)count), dim3(G::NT), 0,
-                       stream, args);
+    hipLaunchKernelGGL((block_gemm_kernel<AT, G, KFENCE, SP>), dim3((unsigned)count), dim3(G::NT),
+                       0, stream, args);
     return hipGetLastError();
 }
 

@@ -60,6 +60,18 @@ void add_instance(std::vector<BlockInstance>& v, int part, int ranks) {
                               AT::KSTEP, &launch_block_gemm<AT, G, KFENCE>, 1, 1});
 }
 
+// A 3D instance whose full-width wave tiles would not fit the register file: the columns
+// are computed in SP passes of BN / SP (the kernel's SP form), each a 3D product of its
+// own with the depth reduction in LDS.
+template <class AT, int PREC, int BM, int BN, int BK, int PM, int PN, int PK, int SP>
+void add_pass_instance(std::vector<BlockInstance>& v, int ranks) {
+    static_assert(BN % SP == 0, "column passes");
+    constexpr int KS = choose_ks<AT, BM, BN / SP, BK, PM, PN, PK>(kStageBudget);
+    using G = BlockGeom<AT, BM, BN / SP, BK, PM, PN, PK, KS>;
+    v.push_back(BlockInstance{PREC, BM, BN, BK, 3, ranks, PM, PN, PK, KS, G::NT, G::LDS,
+                              AT::KSTEP, &launch_block_gemm<AT, G, 0, SP>, 1, 1});
+}
+
 // Resident-repeat form of a 1D or 2D instance: one LDS stage holds the whole K of B (and
 // of A in 2D; in 1D each wave keeps its rows of A in registers), so the repeated
 // products of iters > 1 (the reference's NUM_ITER loop,

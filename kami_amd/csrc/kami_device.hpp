@@ -150,6 +150,26 @@ struct AtomBF16x16 {
     __device__ static inline int out_row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
+// K step 8 on the 16x16x16 instruction, for the 3D split of K = 16 over two depths: lane
+// groups 0 and 1 hold the depth's eight k values; groups 2 and 3 (whose fragment loads
+// fall outside the depth's slice) contribute zeros.
+template <class Base>
+struct AtomK8 : Base {
+    static constexpr int KSTEP = 8;
+    __device__ static inline void mma_sub(typename Base::accv_t& c, const typename Base::frag_t& a,
+                                          const typename Base::frag_t& b, int) {
+        mma(c, a, b);
+    }
+    __device__ static inline void mma(typename Base::accv_t& c, const typename Base::frag_t& a,
+                                      const typename Base::frag_t& b) {
+        const bool live = (__lane_id() >> 5) == 0;
+        const typename Base::frag_t z = {};
+        Base::mma(c, live ? a : z, live ? b : z);
+    }
+};
+using AtomF16x16K8 = AtomK8<AtomF16x16>;
+using AtomBF16x16K8 = AtomK8<AtomBF16x16>;
+
 template <class To, class From>
 __device__ inline To bitcast(const From& f) {
     static_assert(sizeof(To) == sizeof(From), "bitcast size");

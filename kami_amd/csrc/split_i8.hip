@@ -33,6 +33,7 @@
 // whatever order the instruction sums its 64 products in.
 #include <atomic>
 #include <cstdint>
+#include <type_traits>
 
 #include "split_common.hpp"
 
@@ -48,14 +49,21 @@ struct I8Geom {
     static constexpr int WAVES = BM / 16, NT = WAVES * kWave;
     static constexpr int JN = BN / 16, KS = BK / 64;
     static constexpr int FRAG = kWave * 16;  // one wave's 16-byte pieces
-    // Region jn of LDS holds the digit planes of B rows 16 jn .. 16 jn + 15 (plane p,
-    // K step h at (p KS + h) FRAG) and, before they are sliced, the same rows' raw
-    // doubles (K step h, piece i at (8 h + i) FRAG): both take S KS FRAG bytes.
+    // A region holds the digit planes of 16 rows of B (plane p, K step h at
+    // (p KS + h) FRAG) or, before they are sliced, the same rows' raw doubles (K step h,
+    // piece i at (8 h + i) FRAG): both take S KS FRAG bytes.  JN + 1 region slots: the
+    // spare slot lets the next problem's first rows land while every slot of the current
+    // problem is still being read.
     static constexpr int REGION = S * KS * FRAG;
-    static constexpr int EXP_OFF = JN * REGION;
-    static constexpr int LDS = EXP_OFF + (BM + BN) * 4;
-    static_assert(BM % 16 == 0 && BK % 64 == 0 && WAVES >= 1 && WAVES <= 16, "tiling");
+    static constexpr int SLOTS = JN + 1;
+    static constexpr int EXP_OFF = SLOTS * REGION;
+    static constexpr int EXP_BUF = (BM + BN) * 4;  // two buffers: problem parity
+    static constexpr int LDS = EXP_OFF + 2 * EXP_BUF;
+    static_assert(BM % 16 == 0 && BK % 64 == 0 && WAVES >= 2 && WAVES <= 16, "tiling");
     static_assert(S == 8, "a region holds a K step's eight raw pieces or eight planes");
+    // the raw pieces of a region (8 per K step) are loaded by all waves, PPW each
+    static constexpr int PPW = 8 * KS / WAVES;
+    static_assert(PPW >= 1 && PPW * WAVES == 8 * KS, "pieces per wave");
     static_assert(LDS <= 160 * 1024, "LDS budget");
     static_assert(L >= 1 && L <= 2 * S - 1, "levels");
     // at most S products of two digits (|d| <= 128) per k and level
@@ -70,22 +78,22 @@ struct I8Geom {
 
 // The eight balanced byte digits of X = trunc(|x| 2^(62 - E)) sign(x), for |x| < 2^E,
 // packed in one word: byte b is the digit of weight 2^(8b), in [-128, 127], and
-// X = sum_b d_b 2^(8b) exactly.  x is first scaled by 2^-E (exact; it also brings the
-// subnormal entries of a tiny row into the normal range); the 53-bit significand of
-// x 2^-E with its leading bit placed at bit 62 is then shifted right by
-// 1023 - exponent(x 2^-E) >= 1 (a shift of 63 or more gives 0, which covers zero and
-// anything below 2^-62 of the row).  Adding 0x80 to every byte with carries and
-// flipping each byte's top bit back turns the two's complement value into balanced
-// digits.
-__device__ inline uint64_t digit_word(double x, int neg_e) {
-    const uint64_t bits = __builtin_bit_cast(uint64_t, __builtin_amdgcn_ldexp(x, neg_e));
+// X = sum_b d_b 2^(8b) exactly.  With c_e = 1023 + E: the 53-bit significand of x with its
+// leading bit placed at bit 62 is shifted right by min(c_e - exponent field, 63) (63
+// gives 0: bit 63 is clear).  A zero or subnormal x (exponent field 0) is then 0, which
+// is exact when E >= -960 (|x| < 2^-1022 <= 2^(E - 62)); rows with smaller E are scaled
+// by 2^1000 first (slice_step).  Adding 0x80 to every byte with carries and flipping each
+// byte's top bit back turns the two's complement value into balanced digits.
+__device__ inline uint64_t digit_word(double x, uint32_t c_e) {
+    const uint64_t bits = __builtin_bit_cast(uint64_t, x);
     const uint32_t h = (uint32_t)(bits >> 32), l = (uint32_t)bits;
-    const uint32_t ex = (h >> 20) & 0x7ffu;
-    const uint32_t r = min(1023u - ex, 63u);
-    const uint32_t mh = (__builtin_amdgcn_alignbit(h, l, 22) & 0x3fffffffu) | 0x40000000u;
-    const uint64_t X = ((uint64_t)mh << 32 | (uint64_t)(l << 10)) >> r;
-    const uint64_t s = (uint64_t)(int64_t)((int32_t)h >> 31);  // 0 or all ones
-    const uint64_t Y = (X ^ s) + (0x8080808080808080ull + (s & 1));  // +-X + 0x80.. per byte
+    const uint32_t r = min(c_e - ((h >> 20) & 0x7ffu), 63u);
+    const uint64_t M = ((uint64_t)((h & 0xfffffu) | 0x100000u) << 32 | l) << 10;
+    const uint64_t X = M >> r;
+    const int32_t s = (int32_t)h >> 31;  // 0 or -1
+    const uint64_t s64 = (uint64_t)(int64_t)s;
+    // -X = ~X + 1: the +1 rides in the low word of the byte-wise 0x80 offset
+    const uint64_t Y = (X ^ s64) + ((uint64_t)0x80808080u << 32 | (uint32_t)(0x80808080u - s));
     return Y ^ 0x8080808080808080ull;
 }
 
@@ -108,35 +116,63 @@ __device__ inline void transpose4(uint64_t w0, uint64_t w1, uint64_t w2, uint64_
     }
 }
 
+// s_waitcnt immediate that waits for at most n outstanding vector memory operations and
+// leaves the other counters alone (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4,
+// lgkmcnt 11:8).
+constexpr int vmcnt_only(int n) { return ((n >> 4) & 3) << 14 | 0x0F70 | (n & 15); }
+
+// Workgroup barrier for LDS traffic only: this wave's LDS operations complete
+// (lgkmcnt(0)), then s_barrier.  __syncthreads() also fences global memory, which waits
+// for every store and direct-to-LDS load in flight; the data those loads bring is read
+// only by the wave that issued them, after its own vmcnt wait.
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt, expcnt left at their maxima
+    __builtin_amdgcn_s_barrier();
+}
+
+// Rows whose exponent is below this take the scaled path of slice_step.
+constexpr int kTinyRowExp = -960;
+
 // Digit fragments of one 64-wide K step from the lane's eight 16-byte pieces x[i]
 // (doubles k(i, 0), k(i, 1)): frag[p] is the 16-byte MFMA fragment of plane p (p = 0 the
-// most significant digit), slot j = 2 i + q.
-__device__ inline void slice_step(const u4_t (&x)[8], int neg_e, u4_t (&frag)[8]) {
+// most significant digit), slot j = 2 i + q.  `tiny`: some row of the wave has
+// E < -960; its elements are scaled by 2^1000 (exact) so that their subnormals are
+// normal, with E raised by the same amount.
+__device__ inline void slice_step(const u4_t (&x)[8], int E, bool tiny, u4_t (&frag)[8]) {
+    const int sc = (tiny && E < kTinyRowExp) ? 1000 : 0;
+    const uint32_t c_e = (uint32_t)(1023 + E + sc);
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-        const d2_t x0 = bitcast<d2_t>(x[2 * m]), x1 = bitcast<d2_t>(x[2 * m + 1]);
+        d2_t x0 = bitcast<d2_t>(x[2 * m]), x1 = bitcast<d2_t>(x[2 * m + 1]);
+        if (tiny) {
+            x0[0] = __builtin_amdgcn_ldexp(x0[0], sc);
+            x0[1] = __builtin_amdgcn_ldexp(x0[1], sc);
+            x1[0] = __builtin_amdgcn_ldexp(x1[0], sc);
+            x1[1] = __builtin_amdgcn_ldexp(x1[1], sc);
+        }
         uint32_t out[8];
-        transpose4(digit_word(x0[0], neg_e), digit_word(x0[1], neg_e), digit_word(x1[0], neg_e),
-                   digit_word(x1[1], neg_e), out);
+        transpose4(digit_word(x0[0], c_e), digit_word(x0[1], c_e), digit_word(x1[0], c_e),
+                   digit_word(x1[1], c_e), out);
 #pragma unroll
         for (int b = 0; b < 8; ++b) frag[7 - b][m] = out[b];
     }
 }
 
-// Exponent e with |x| < 2^e over a row: the largest upper word of |x| over the lane's
-// values and the row's four lane groups (lanes l, l ^ 16, l ^ 32, l ^ 48), completed
-// with all-ones lower bits so that frexp also covers subnormal rows (an all-zero row
-// gives the exponent of the smallest subnormal range, and digits 0).
+// Exponent E with |x| < 2^E over a row: frexp of the largest |x| over the lane's values
+// and the row's four lane groups (lanes l, l ^ 16, l ^ 32, l ^ 48); 0 for a zero row.
 template <int KS>
 __device__ inline int row_exponent(const u4_t (&x)[KS][8]) {
-    uint32_t m = 0;
+    double m = 0.0;
 #pragma unroll
     for (int h = 0; h < KS; ++h)
 #pragma unroll
-        for (int i = 0; i < 8; ++i) m = max(m, max(x[h][i][1] & 0x7fffffffu, x[h][i][3] & 0x7fffffffu));
-    m = max(m, (uint32_t)__shfl_xor((int)m, 16, 64));
-    m = max(m, (uint32_t)__shfl_xor((int)m, 32, 64));
-    return __builtin_amdgcn_frexp_exp(__builtin_bit_cast(double, ((uint64_t)m << 32) | 0xffffffffu));
+        for (int i = 0; i < 8; ++i) {
+            const d2_t d = bitcast<d2_t>(x[h][i]);
+            m = __builtin_fmax(m, __builtin_fmax(__builtin_fabs(d[0]), __builtin_fabs(d[1])));
+        }
+    m = __builtin_fmax(m, __shfl_xor(m, 16, 64));
+    m = __builtin_fmax(m, __shfl_xor(m, 32, 64));
+    return __builtin_amdgcn_frexp_exp(m);
 }
 
 // Persistent over problems p = blockIdx.x, + gridDim.x, ...  REPEAT (iters > 1) repeats
@@ -146,20 +182,26 @@ __device__ inline int row_exponent(const u4_t (&x)[KS][8]) {
 // less than 2^31 bytes).
 //
 // Pipeline.  The raw rows of A, of B and the wave's A digits do not fit in registers
-// together beside the level sums, so the next problem's B rows go straight to LDS: once
-// every wave has finished column block jn of the current products (a barrier per block),
-// wave jn loads B rows 16 jn .. 16 jn + 15 of the next problem into region jn with
-// direct-to-LDS loads.  The next problem's A rows are loaded into registers right after
-// the current ones are sliced.  Slicing a problem: A from registers into the wave's
-// fragments, then the wave's raw B rows from its own region into registers and their
-// digit planes back into the same region.
+// together beside the level sums, so the next problem's B rows go straight to LDS with
+// direct-to-LDS loads, 16 rows (one region) at a time, every wave loading a share of each
+// region.  With i the workgroup's problem count, region jn of problem i sits in slot
+// (jn - i) mod (JN + 1): region 0 of the next problem takes the slot the current one
+// leaves free (loaded as the products start), and region jn of the next problem takes
+// the slot of the current one's region jn - 1 once every wave has finished that column
+// block (a barrier per block).  The next problem's A rows are loaded into registers
+// after the current problem is sliced.  Slicing a problem: A from registers into the
+// wave's fragments; then, once every wave's loads have landed (a barrier), the wave's
+// raw B rows from region `wave` into registers and their digit planes back into the same
+// region.  Barriers wait for LDS operations only (lds_barrier), not for the loads in
+// flight.  Exponents alternate between two buffers, so the last column block needs no
+// barrier.
 template <class G, bool REPEAT>
 __global__ void __launch_bounds__(G::NT)
 split_i8_kernel(SplitArgs args, int64_t count) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[G::LDS];
-    int* const expo = reinterpret_cast<int*>(smem + G::EXP_OFF);  // [BM] of A, then [BN] of B
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     const int r = lane & 15, grp = lane >> 4;
     const int row = wave * 16 + r;  // operand row this lane slices, of A and of B
     const int64_t grid = gridDim.x;
@@ -168,7 +210,6 @@ split_i8_kernel(SplitArgs args, int64_t count) {
     // byte offset of the lane's output (row 16 wave + 4 grp + v, column 16 jn + r)
     const uint32_t c_off = (uint32_t)(((wave * 16 + 4 * grp) * args.ldc + r) * 8);
     const uint32_t c_row = (uint32_t)(args.ldc * 8);
-    unsigned char* const region = smem + wave * G::REGION;  // this wave's B rows
 
     u4_t xa[G::KS][8];
     // A rows of problem q through a buffer descriptor; with `live` false the descriptor
@@ -184,81 +225,110 @@ split_i8_kernel(SplitArgs args, int64_t count) {
                 xa[h][i] = __builtin_bit_cast(u4_t, __builtin_amdgcn_raw_buffer_load_b128(
                                                         rs, (int)(a_off + 8 * (64 * h + 8 * i)), 0, 0));
     };
-    // the wave's B rows of problem q into its region, piece (h, i) of lane l at
-    // (8 h + i) FRAG + 16 l
-    auto load_b_lds = [&](int64_t q) {
-        const unsigned char* src = reinterpret_cast<const unsigned char*>(args.B + q * args.sb);
-#pragma unroll
-        for (int h = 0; h < G::KS; ++h)
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-                __builtin_amdgcn_global_load_lds(
-                    src + b_off + 8 * (64 * h + 8 * i),
-                    (__attribute__((address_space(3))) void*)(region + (8 * h + i) * G::FRAG), 16, 0,
-                    0);
+    auto slot_of = [&](int64_t i, int jn) {  // region jn of the workgroup's i-th problem
+        return smem + (int)(((int64_t)jn - i % G::SLOTS + G::SLOTS) % G::SLOTS) * G::REGION;
     };
-    // B fragment of plane p, column block jn, K step h
-    auto bfrag_at = [&](int p, int jn, int h) {
-        return smem + jn * G::REGION + (p * G::KS + h) * G::FRAG + 16 * lane;
+    // Raw B rows 16 reg .. 16 reg + 15 of problem q into LDS region `dst` with
+    // direct-to-LDS buffer loads: piece pc = 8 h + i (K step h, lane l: doubles
+    // 64 h + 8 i + 2 (l / 16) + {0, 1} of row 16 reg + l mod 16) lands at pc FRAG + 16 l.
+    // This wave issues pieces [first, first + n).  `live` false: a rangeless descriptor
+    // (no memory traffic), so every wave issues the same loads on every path.
+    const uint32_t b_lane = (uint32_t)((r * args.ldb + 2 * grp) * 8);
+    const uint32_t b_rows16 = (uint32_t)(16 * args.ldb * 8);
+    auto load_b_piece = [&](__amdgpu_buffer_rsrc_t rs, uint32_t off, unsigned char* dst, int pc) {
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            rs, (__attribute__((address_space(3))) void*)(dst + pc * G::FRAG), 16,
+            (int)(off + 8 * (64 * (pc >> 3) + 8 * (pc & 7))), 0, 0, 0);
+    };
+    auto b_rsrc = [&](int64_t q, bool live) {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(args.B + q * args.sb), (short)0,
+                                                 live ? 0x7FFFFFF0 : 0, 0x00020000);
+    };
+    // this wave's share of region `reg` of problem q: pieces wave PPW .. wave PPW + PPW - 1
+    auto load_b_share = [&](int64_t q, bool live, int reg, unsigned char* dst) {
+        const __amdgpu_buffer_rsrc_t rs = b_rsrc(q, live);
+        const uint32_t off = b_lane + (uint32_t)reg * b_rows16;
+#pragma unroll
+        for (int t = 0; t < G::PPW; ++t) load_b_piece(rs, off, dst, wave * G::PPW + t);
     };
 
     int64_t p = blockIdx.x;
     if (p >= count) return;  // uniform
-    load_b_lds(p);
+    {  // the first problem: each wave loads its own rows
+        const __amdgpu_buffer_rsrc_t rs = b_rsrc(p, true);
+#pragma unroll
+        for (int pc = 0; pc < 8 * G::KS; ++pc)
+            load_b_piece(rs, b_lane + (uint32_t)wave * b_rows16, slot_of(0, wave), pc);
+    }
     load_a(p, true);
     u4_t ad[G::S][G::KS];  // the wave's A digit fragments
 
-    for (; p < count; p += grid) {
+    for (int64_t i = 0; p < count; p += grid, ++i) {
         const bool next = p + grid < count;
+        int* const expo = reinterpret_cast<int*>(smem + G::EXP_OFF + (i & 1) * G::EXP_BUF);
+        unsigned char* const region = slot_of(i, wave);  // this wave's B rows
         // ---- A: exponents and digits into registers ----
-        const int ea = row_exponent<G::KS>(xa);
-        expo[row] = ea;  // the row's four lane groups write the same value
+        {
+            const int ea = row_exponent<G::KS>(xa);
+            expo[row] = ea;  // the row's four lane groups write the same value
+            const bool tiny = __builtin_amdgcn_ballot_w64(ea < kTinyRowExp) != 0;
 #pragma unroll
-        for (int h = 0; h < G::KS; ++h) {
-            u4_t f[8];
-            slice_step(xa[h], -ea, f);
+            for (int h = 0; h < G::KS; ++h) {
+                u4_t f[8];
+                slice_step(xa[h], ea, tiny, f);
 #pragma unroll
-            for (int q = 0; q < G::S; ++q) ad[q][h] = f[q];
-            __builtin_amdgcn_sched_barrier(0);
+                for (int q = 0; q < G::S; ++q) ad[q][h] = f[q];
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
         // ---- B: the wave's raw rows from its region, digit planes back into it ----
-        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the region's loads have landed
+        // every wave's direct-to-LDS loads have landed
+        __builtin_amdgcn_s_waitcnt(vmcnt_only(0));
+        lds_barrier();
         {
             u4_t xb[G::KS][8];
 #pragma unroll
             for (int h = 0; h < G::KS; ++h)
 #pragma unroll
-                for (int i = 0; i < 8; ++i) xb[h][i] = load_raw<16>(region + (8 * h + i) * G::FRAG + 16 * lane);
+                for (int q = 0; q < 8; ++q) xb[h][q] = load_raw<16>(region + (8 * h + q) * G::FRAG + 16 * lane);
             const int eb = row_exponent<G::KS>(xb);
             expo[G::BM + row] = eb;
+            const bool tiny = __builtin_amdgcn_ballot_w64(eb < kTinyRowExp) != 0;
 #pragma unroll
             for (int h = 0; h < G::KS; ++h) {
                 u4_t f[8];
-                slice_step(xb[h], -eb, f);
+                slice_step(xb[h], eb, tiny, f);
 #pragma unroll
-                for (int q = 0; q < G::S; ++q) store_raw<16>(bfrag_at(q, wave, h), f[q]);
+                for (int q = 0; q < G::S; ++q)
+                    store_raw<16>(region + (q * G::KS + h) * G::FRAG + 16 * lane, f[q]);
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-        // the next problem's A rows are in flight during the products
+        // the next problem's A rows: in flight during the products
         load_a(next ? p + grid : p, next);
-        __syncthreads();
+        lds_barrier();
+        // region 0 of the next problem goes to the slot this problem leaves free
+        load_b_share(next ? p + grid : p, next, 0, slot_of(i + 1, 0));
 
         // ---- level sums on int8 MFMA, Horner in FP64, store ----
         unsigned char* const Cp = reinterpret_cast<unsigned char*>(args.C + p * args.sc);
         int erow[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) erow[v] = expo[wave * 16 + 4 * grp + v];
-#pragma unroll 1
+#pragma unroll
         for (int jn = 0; jn < G::JN; ++jn) {
             const int ecol = expo[G::BM + jn * 16 + r];
+            const unsigned char* const bsrc = slot_of(i, jn) + 16 * lane;
+            auto bfrag = [&](int pb, int h) {
+                return bitcast<i4_t>(load_raw<16>(bsrc + (pb * G::KS + h) * G::FRAG));
+            };
             double sum[4] = {0.0, 0.0, 0.0, 0.0};
             for (int it = 0; it < (REPEAT ? args.iters : 1); ++it) {
                 i4_t T[G::L];
 #pragma unroll
                 for (int l = 0; l < G::L; ++l) T[l] = i4_t{0, 0, 0, 0};
                 // B fragments one ahead of the products that use them
-                i4_t bf = bitcast<i4_t>(load_raw<16>(bfrag_at(0, jn, 0)));
+                i4_t bf = bfrag(0, 0);
 #pragma unroll
                 for (int h = 0; h < G::KS; ++h)
 #pragma unroll
@@ -266,9 +336,7 @@ split_i8_kernel(SplitArgs args, int64_t count) {
                         const bool last = h + 1 == G::KS && (pb + 1 == G::L || pb + 1 == G::S);
                         i4_t bn = bf;
                         if (!last)
-                            bn = bitcast<i4_t>(load_raw<16>(
-                                pb + 1 < G::L && pb + 1 < G::S ? bfrag_at(pb + 1, jn, h)
-                                                                : bfrag_at(0, jn, h + 1)));
+                            bn = (pb + 1 < G::L && pb + 1 < G::S) ? bfrag(pb + 1, h) : bfrag(0, h + 1);
 #pragma unroll
                         for (int pa = 0; pa + pb < G::L && pa < G::S; ++pa)
                             T[pa + pb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
@@ -283,6 +351,7 @@ split_i8_kernel(SplitArgs args, int64_t count) {
                     for (int l = G::L - 2; l >= 0; --l)
                         acc = __builtin_fma(acc, 1.0 / 256.0, (double)T[l][v]);
                     sum[v] += acc;
+                    __builtin_amdgcn_sched_barrier(0);  // one output's conversions at a time
                 }
             }
 #pragma unroll
@@ -292,11 +361,290 @@ split_i8_kernel(SplitArgs args, int64_t count) {
                 if (args.beta != 0.0) val += args.beta * (*dst);
                 *dst = val;
             }
-            // every wave is done with region jn: its wave loads the next problem's rows
-            __syncthreads();
-            if (next && wave == jn) load_b_lds(p + grid);
+            if (jn + 1 < G::JN) {
+                // every wave is done with column block jn: its slot takes region jn + 1 of
+                // the next problem
+                lds_barrier();
+                load_b_share(next ? p + grid : p, next, jn + 1, slot_of(i + 1, jn + 1));
+            }
         }
     }
+}
+
+// ---- streaming form (iters = 1): B sliced column block by column block ----
+//
+// Column block c of a problem needs only the digit planes of B rows 16c .. 16c + 15
+// (region c), so B is streamed through LDS one region at a time, in a pipeline over the
+// workgroup's consecutive regions t = JN k + c (problem k, block c): while the waves
+// multiply region t, they slice region t + 1 and take the row maxima of region t + 2,
+// whose raw rows were loaded D - 2 columns earlier; every wave works on its own share of
+// each region (two 1 KB pieces: K step h = wave / 4 and four consecutive k of every
+// lane's fragment slot group, so the slicing of a share writes one dword of each plane
+// fragment).  The row exponent of a region is the maximum of the waves' partial row
+// maxima (one LDS round through a barrier).  VALU slicing and MFMA products of the same
+// column run in the same stretch of instructions, so they overlap; A is still sliced
+// into registers at the first column of each problem.  LDS: D raw regions, two plane
+// regions, partial maxima and exponents, ~100 KB for 128^3.
+template <class G>
+struct I8Stream {
+    static constexpr int D = 4;  // raw regions held: t + 1 .. t + D
+    static constexpr int RAW_OFF = 0;
+    static constexpr int PLN_OFF = D * G::REGION;
+    static constexpr int PMAX_OFF = PLN_OFF + 2 * G::REGION;     // int [2][WAVES][16]
+    static constexpr int EXPB_OFF = PMAX_OFF + 2 * G::WAVES * 16 * 4;  // int [3][16]
+    static constexpr int EXPA_OFF = EXPB_OFF + 3 * 16 * 4;       // int [BM]
+    static constexpr int LDS = EXPA_OFF + G::BM * 4;
+    // a wave's share of a region is pieces 2 wave, 2 wave + 1 of the region's 8 KS
+    static_assert(G::PPW == 2 && G::BM == G::BK, "two pieces per wave and region");
+    static_assert(LDS <= 160 * 1024, "LDS budget");
+};
+
+template <class G>
+__global__ void __launch_bounds__(G::NT)
+split_i8_stream_kernel(SplitArgs args, int64_t count) {
+    using St = I8Stream<G>;
+    __shared__ __attribute__((aligned(16))) unsigned char smem[St::LDS];
+    int* const pmax = reinterpret_cast<int*>(smem + St::PMAX_OFF);
+    int* const expb = reinterpret_cast<int*>(smem + St::EXPB_OFF);
+    int* const expa = reinterpret_cast<int*>(smem + St::EXPA_OFF);
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int r = lane & 15, grp = lane >> 4;
+    const int row = wave * 16 + r;
+    const int64_t first = blockIdx.x, grid = gridDim.x;
+    if (first >= count) return;  // uniform
+    const int64_t nreg = (count - first + grid - 1) / grid * G::JN;  // this workgroup's regions
+    const uint32_t a_off = (uint32_t)((row * args.lda + 2 * grp) * 8);
+    const uint32_t c_off = (uint32_t)(((wave * 16 + 4 * grp) * args.ldc + r) * 8);
+    const uint32_t c_row = (uint32_t)(args.ldc * 8);
+    // the wave's share: K step sh, pieces si and si + 1 (k = 64 sh + 8 (si + q) + 2 grp + {0, 1})
+    const int sh = (2 * wave) >> 3, si = (2 * wave) & 7;
+    const uint32_t b_lane = (uint32_t)((r * args.ldb + 64 * sh + 8 * si + 2 * grp) * 8);
+    const uint32_t b_rows16 = (uint32_t)(16 * args.ldb * 8);
+    auto prob_of = [&](int64_t reg) { return first + reg / G::JN * grid; };
+
+    u4_t xa[G::KS][8];
+    auto load_a = [&](int64_t q, bool live) {
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<double*>(args.A + q * args.sa), (short)0, live ? 0x7FFFFFF0 : 0, 0x00020000);
+#pragma unroll
+        for (int h = 0; h < G::KS; ++h)
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                xa[h][i] = __builtin_bit_cast(u4_t, __builtin_amdgcn_raw_buffer_load_b128(
+                                                        rs, (int)(a_off + 8 * (64 * h + 8 * i)), 0, 0));
+    };
+    // this wave's share of region reg into raw slot reg mod D (a rangeless descriptor past
+    // the last region: no memory traffic)
+    auto dma = [&](int64_t reg) {
+        const bool live = reg < nreg;
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<double*>(args.B + (live ? prob_of(reg) : first) * args.sb), (short)0,
+            live ? 0x7FFFFFF0 : 0, 0x00020000);
+        const uint32_t off = b_lane + (uint32_t)(reg % G::JN) * b_rows16;
+        unsigned char* dst = smem + St::RAW_OFF + (int)(reg % St::D) * G::REGION + 2 * wave * G::FRAG;
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                rs, (__attribute__((address_space(3))) void*)(dst + q * G::FRAG), 16,
+                (int)(off + 64 * q), 0, 0, 0);
+    };
+    auto raw_share = [&](int64_t reg, u4_t (&x)[2]) {
+        const unsigned char* src =
+            smem + St::RAW_OFF + (int)(reg % St::D) * G::REGION + 2 * wave * G::FRAG + 16 * lane;
+        x[0] = load_raw<16>(src);
+        x[1] = load_raw<16>(src + G::FRAG);
+    };
+    // partial row maxima of the share (as frexp exponents: the exponent of a maximum is
+    // the maximum of the exponents)
+    auto pmax_write = [&](int64_t reg, const u4_t (&x)[2]) {
+        const d2_t x0 = bitcast<d2_t>(x[0]), x1 = bitcast<d2_t>(x[1]);
+        double m = __builtin_fmax(__builtin_fmax(__builtin_fabs(x0[0]), __builtin_fabs(x0[1])),
+                                  __builtin_fmax(__builtin_fabs(x1[0]), __builtin_fabs(x1[1])));
+        m = __builtin_fmax(m, __shfl_xor(m, 16, 64));
+        m = __builtin_fmax(m, __shfl_xor(m, 32, 64));
+        pmax[((int)(reg & 1) * G::WAVES + wave) * 16 + r] = __builtin_amdgcn_frexp_exp(m);
+    };
+    // digits of the share into plane slot reg mod 2: dword si / 2 of every lane's
+    // fragment of plane p, K step sh
+    auto slice_share = [&](int64_t reg, const u4_t (&x)[2]) {
+        const int* pm = pmax + (int)(reg & 1) * G::WAVES * 16 + r;
+        int E = pm[0];
+#pragma unroll
+        for (int w = 1; w < G::WAVES; ++w) E = max(E, pm[w * 16]);
+        if (wave == 0 && grp == 0) expb[(int)(reg % 3) * 16 + r] = E;
+        d2_t x0 = bitcast<d2_t>(x[0]), x1 = bitcast<d2_t>(x[1]);
+        int sc = 0;
+        if (__builtin_amdgcn_ballot_w64(E < kTinyRowExp) != 0) {  // rare: tiny rows
+            sc = E < kTinyRowExp ? 1000 : 0;
+            x0[0] = __builtin_amdgcn_ldexp(x0[0], sc);
+            x0[1] = __builtin_amdgcn_ldexp(x0[1], sc);
+            x1[0] = __builtin_amdgcn_ldexp(x1[0], sc);
+            x1[1] = __builtin_amdgcn_ldexp(x1[1], sc);
+        }
+        const uint32_t c_e = (uint32_t)(1023 + E + sc);
+        uint32_t out[8];
+        transpose4(digit_word(x0[0], c_e), digit_word(x0[1], c_e), digit_word(x1[0], c_e),
+                   digit_word(x1[1], c_e), out);
+        unsigned char* dst = smem + St::PLN_OFF + (int)(reg & 1) * G::REGION + sh * G::FRAG +
+                             16 * lane + 4 * (si >> 1);
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+            *reinterpret_cast<uint32_t*>(dst + (7 - b) * G::KS * G::FRAG) = out[b];
+    };
+
+    // ---- prologue: regions 0 .. D-1 and the first A rows in flight; maxima of regions
+    //      0 and 1, digits of region 0 ----
+    for (int d = 0; d < St::D; ++d) dma(d);
+    load_a(first, true);
+    __builtin_amdgcn_s_waitcnt(vmcnt_only(0));
+    {
+        u4_t x[2];
+        raw_share(0, x);
+        pmax_write(0, x);
+        raw_share(1, x);
+        pmax_write(1, x);
+        lds_barrier();
+        raw_share(0, x);
+        slice_share(0, x);
+        lds_barrier();
+    }
+
+    u4_t ad[G::S][G::KS];
+    int erow[4];
+    for (int64_t t = 0; t < nreg; ++t) {
+        const int c = (int)(t % G::JN);
+        const int64_t p = prob_of(t);
+        if (c == 0) {
+            // ---- a new problem: A exponents and digits into registers ----
+            const int ea = row_exponent<G::KS>(xa);
+            expa[row] = ea;
+            const bool tiny = __builtin_amdgcn_ballot_w64(ea < kTinyRowExp) != 0;
+#pragma unroll
+            for (int h = 0; h < G::KS; ++h) {
+                u4_t f[8];
+                slice_step(xa[h], ea, tiny, f);
+#pragma unroll
+                for (int q = 0; q < G::S; ++q) ad[q][h] = f[q];
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) erow[v] = expa[wave * 16 + 4 * grp + v];  // own rows
+            const bool more = p + grid < count;
+            load_a(more ? p + grid : p, more);
+        }
+        // ---- B pipeline: maxima of region t + 2, digits of region t + 1 ----
+        // this wave's loads of region t + 2 (issued two columns ago) have landed: in
+        // order, only the later ones may be in flight -- per column 2 region loads and 4
+        // stores, plus the A loads issued at a problem's first column
+        if (args.beta != 0.0) {
+            __builtin_amdgcn_s_waitcnt(vmcnt_only(0));  // C loads change the counts
+        } else if (c <= 1) {
+            __builtin_amdgcn_s_waitcnt(vmcnt_only(10 + 8 * G::KS));
+        } else {
+            __builtin_amdgcn_s_waitcnt(vmcnt_only(10));
+        }
+        dma(t + St::D);  // into the slot region t left
+        u4_t x2[2], x1[2];
+        raw_share(t + 2, x2);
+        raw_share(t + 1, x1);
+        // row exponents of region t + 1: the largest of the waves' partial maxima
+        int E;
+        {
+            const int* pm = pmax + (int)((t + 1) & 1) * G::WAVES * 16 + r;
+            E = pm[0];
+#pragma unroll
+            for (int w = 1; w < G::WAVES; ++w) E = max(E, pm[w * 16]);
+        }
+        expb[(int)((t + 1) % 3) * 16 + r] = E;  // every wave and lane group: the same value
+        const int ecol = expb[(int)(t % 3) * 16 + r];
+        const unsigned char* const bsrc = smem + St::PLN_OFF + (int)(t & 1) * G::REGION + 16 * lane;
+        auto bfrag = [&](int pb, int h) {
+            return bitcast<i4_t>(load_raw<16>(bsrc + (pb * G::KS + h) * G::FRAG));
+        };
+        i4_t T[G::L];
+        // One straight stretch of code: the partial maxima of region t + 2, the digits of
+        // region t + 1 (VALU) and the level products of region t (MFMA), independent of
+        // each other; the scheduling groups at its end interleave them so that the VALU
+        // work issues while the MFMAs run.  TINY: some row of region t + 1 has
+        // E < -960 and takes the scaled slicing.
+        auto column = [&](auto tiny_tag) {
+            constexpr bool TINY = decltype(tiny_tag)::value;
+#pragma unroll
+            for (int l = 0; l < G::L; ++l) T[l] = i4_t{0, 0, 0, 0};
+            i4_t bf = bfrag(0, 0);
+#pragma unroll
+            for (int h = 0; h < G::KS; ++h)
+#pragma unroll
+                for (int pb = 0; pb < G::L && pb < G::S; ++pb) {
+                    const bool last = h + 1 == G::KS && (pb + 1 == G::L || pb + 1 == G::S);
+                    i4_t bn = bf;
+                    if (!last) bn = (pb + 1 < G::L && pb + 1 < G::S) ? bfrag(pb + 1, h) : bfrag(0, h + 1);
+#pragma unroll
+                    for (int pa = 0; pa + pb < G::L && pa < G::S; ++pa)
+                        T[pa + pb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bitcast<i4_t>(ad[pa][h]),
+                                                                           bf, T[pa + pb], 0, 0, 0);
+                    bf = bn;
+                }
+            // the LDS writes of the slicing come after the B fragment reads of the products
+            // (they may not be reordered across them), its arithmetic is free to move up
+            pmax_write(t + 2, x2);
+            {
+                d2_t a0 = bitcast<d2_t>(x1[0]), a1 = bitcast<d2_t>(x1[1]);
+                int sc = 0;
+                if constexpr (TINY) {
+                    sc = E < kTinyRowExp ? 1000 : 0;
+                    a0[0] = __builtin_amdgcn_ldexp(a0[0], sc);
+                    a0[1] = __builtin_amdgcn_ldexp(a0[1], sc);
+                    a1[0] = __builtin_amdgcn_ldexp(a1[0], sc);
+                    a1[1] = __builtin_amdgcn_ldexp(a1[1], sc);
+                }
+                const uint32_t c_e = (uint32_t)(1023 + E + sc);
+                uint32_t out[8];
+                transpose4(digit_word(a0[0], c_e), digit_word(a0[1], c_e), digit_word(a1[0], c_e),
+                           digit_word(a1[1], c_e), out);
+                unsigned char* dst = smem + St::PLN_OFF + (int)((t + 1) & 1) * G::REGION +
+                                     sh * G::FRAG + 16 * lane + 4 * (si >> 1);
+#pragma unroll
+                for (int b = 0; b < 8; ++b)
+                    *reinterpret_cast<uint32_t*>(dst + (7 - b) * G::KS * G::FRAG) = out[b];
+            }
+            // one MFMA, then two VALU instructions, while the VALU work lasts
+#pragma unroll
+            for (int q = 0; q < G::KS * G::products(); ++q) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
+            }
+        };
+        if (__builtin_amdgcn_ballot_w64(E < kTinyRowExp) != 0)
+            column(std::true_type{});
+        else
+            column(std::false_type{});
+        unsigned char* const Cp = reinterpret_cast<unsigned char*>(args.C + p * args.sc);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            double acc = (double)T[G::L - 1][v];
+#pragma unroll
+            for (int l = G::L - 2; l >= 0; --l) acc = __builtin_fma(acc, 1.0 / 256.0, (double)T[l][v]);
+            double* dst = reinterpret_cast<double*>(Cp + c_off + v * c_row + c * 128);
+            double val = __builtin_amdgcn_ldexp(acc, erow[v] + ecol - 12) * args.alpha;
+            if (args.beta != 0.0) val += args.beta * (*dst);
+            *dst = val;
+        }
+        lds_barrier();
+    }
+}
+
+template <class G>
+hipError_t launch_split_i8_stream(const SplitArgs& a, int64_t count, hipStream_t s) {
+    static std::atomic<int> cache[kMaxCachedDevices];
+    int slots = 0;
+    const hipError_t e = resident_workgroups(split_i8_stream_kernel<G>, G::NT, cache, &slots);
+    if (e != hipSuccess) return e;
+    const int64_t grid = std::min<int64_t>(count, (int64_t)slots);
+    hipLaunchKernelGGL((split_i8_stream_kernel<G>), dim3((unsigned)grid), dim3(G::NT), 0, s, a, count);
+    return hipGetLastError();
 }
 
 template <class G, bool REPEAT>
@@ -314,7 +662,7 @@ hipError_t launch_split_i8_form(const SplitArgs& a, int64_t count, hipStream_t s
 template <class G>
 hipError_t launch_split_i8(const SplitArgs& a, int64_t count, hipStream_t s) {
     return a.iters > 1 ? launch_split_i8_form<G, true>(a, count, s)
-                       : launch_split_i8_form<G, false>(a, count, s);
+                       : launch_split_i8_stream<G>(a, count, s);
 }
 
 template <int BM, int BK, int L>

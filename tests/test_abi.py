@@ -62,8 +62,17 @@ def test_unsupported_shape_is_enotsup():
     d = K.make_desc(48, 48, 48, K.PREC_F64)
     cfg = _abi.GemmConfig()
     assert K.lib().kami_block_gemm_query(ctypes.byref(d), ctypes.byref(cfg)) == _abi.ENOTSUP
-    d = K.make_desc(256, 256, 256, K.PREC_F16_F32, partition="3d")
+    d = K.make_desc(256, 256, 256, K.PREC_F16_F32, partition="3d", ranks=4)  # 64 waves
     assert K.lib().kami_block_gemm_query(ctypes.byref(d), ctypes.byref(cfg)) == _abi.ENOTSUP
+
+
+@pytest.mark.parametrize("prec", [K.PREC_F16_F32, K.PREC_BF16_F32])
+@pytest.mark.parametrize("shape", [(192, 192, 192), (256, 256, 256), (256, 256, 16), (256, 256, 32)])
+def test_3d_at_wide_shapes_resolves_to_2x2x2_waves(prec, shape):
+    """3D at M = N >= 192 runs 2 x 2 x 2 waves over column passes of the output."""
+    cfg = K.query(*shape, prec, partition="3d")
+    assert (cfg["waves_m"], cfg["waves_n"], cfg["waves_k"], cfg["threads"]) == (2, 2, 2, 512)
+    assert cfg["partition"] == K.PART_3D and cfg["lds_bytes"] <= 160 * 1024
 
 
 @pytest.mark.parametrize("prec,s,part,waves", [
