@@ -34,6 +34,7 @@
 #include <atomic>
 #include <cstdint>
 #include <type_traits>
+#include <utility>
 
 #include "split_common.hpp"
 
@@ -382,27 +383,62 @@ split_i8_kernel(SplitArgs args, int64_t count) {
 // lane's fragment slot group, so the slicing of a share writes one dword of each plane
 // fragment).  The row exponent of a region is the maximum of the waves' partial row
 // maxima (one LDS round through a barrier).  VALU slicing and MFMA products of the same
-// column run in the same stretch of instructions, so they overlap; A is still sliced
-// into registers at the first column of each problem.  LDS: D raw regions, two plane
-// regions, partial maxima and exponents, ~100 KB for 128^3.
-template <class G>
+// column run in the same stretch of instructions, so they overlap.  A is sliced into
+// registers at the first column of each problem; the next problem's A rows arrive in
+// registers two or three 16-byte pieces per column over columns 0 .. JN - 2, so every
+// column issues about the same number of loads and the memory system sees an even
+// stream (a whole problem's A at once is 128 KB per workgroup, and vector memory
+// operations complete in order, so the region waits of the next columns would wait for
+// all of it).  LDS: D raw regions, two plane regions, partial maxima and exponents
+// (~100 KB for 128^3 with D = 4).
+template <class G, int D_>
 struct I8Stream {
-    static constexpr int D = 4;  // raw regions held: t + 1 .. t + D
+    static constexpr int D = D_;  // raw regions held: t + 1 .. t + D
     static constexpr int RAW_OFF = 0;
     static constexpr int PLN_OFF = D * G::REGION;
     static constexpr int PMAX_OFF = PLN_OFF + 2 * G::REGION;     // int [2][WAVES][16]
-    static constexpr int EXPB_OFF = PMAX_OFF + 2 * G::WAVES * 16 * 4;  // int [3][16]
-    static constexpr int EXPA_OFF = EXPB_OFF + 3 * 16 * 4;       // int [BM]
+    static constexpr int EXPB_OFF = PMAX_OFF + 2 * G::WAVES * 16 * 4;  // int [2][16]
+    static constexpr int EXPA_OFF = EXPB_OFF + 2 * 16 * 4;       // int [BM]
     static constexpr int LDS = EXPA_OFF + G::BM * 4;
     // a wave's share of a region is pieces 2 wave, 2 wave + 1 of the region's 8 KS
     static_assert(G::PPW == 2 && G::BM == G::BK, "two pieces per wave and region");
+    static_assert(D >= 3, "a region's raw rows land at least one column before its maxima");
     static_assert(LDS <= 160 * 1024, "LDS budget");
+    // The lane's 8 KS pieces of the next problem's A rows, spread over columns
+    // 0 .. JN - 2 (piece pc: K step pc / 8, fragment slot group pc % 8): at least two
+    // columns pass between a piece's load and its use at the next problem's column 0.
+    static constexpr int PIECES = 8 * G::KS;
+    static_assert(G::JN >= 3, "A pieces spread over at least two columns");
+    static constexpr int a_count(int c) {
+        return c >= G::JN - 1 ? 0 : PIECES / (G::JN - 1) + (c < PIECES % (G::JN - 1) ? 1 : 0);
+    }
+    static constexpr int a_first(int c) {
+        int s = 0;
+        for (int i = 0; i < c; ++i) s += a_count(i);
+        return s;
+    }
+    static_assert(a_first(G::JN) == PIECES, "every piece is loaded once");
+    // Column c issues, in order: its region load (2 operations), a_count(c) A loads and 4
+    // stores.  Region t + 2 was loaded first thing at column t + 2 - D, so at column t it
+    // has landed once at most the operations issued after it are outstanding.
+    static constexpr int col(int c) { return (c % G::JN + G::JN) % G::JN; }
+    static constexpr int region_wait(int c) {
+        int n = a_count(col(c + 2 - D)) + 4;
+        for (int j = c + 3 - D; j < c; ++j) n += 2 + a_count(col(j)) + 4;
+        return n;
+    }
+    static_assert(region_wait(0) < 64 && region_wait(1) < 64 && region_wait(2) < 64, "vmcnt range");
 };
 
-template <class G>
+template <class F, int... I>
+__device__ inline void unroll_seq(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+
+template <class G, int D, bool BETA>
 __global__ void __launch_bounds__(G::NT)
 split_i8_stream_kernel(SplitArgs args, int64_t count) {
-    using St = I8Stream<G>;
+    using St = I8Stream<G, D>;
     __shared__ __attribute__((aligned(16))) unsigned char smem[St::LDS];
     int* const pmax = reinterpret_cast<int*>(smem + St::PMAX_OFF);
     int* const expb = reinterpret_cast<int*>(smem + St::EXPB_OFF);
@@ -414,57 +450,64 @@ split_i8_stream_kernel(SplitArgs args, int64_t count) {
     const int row = wave * 16 + r;
     const int64_t first = blockIdx.x, grid = gridDim.x;
     if (first >= count) return;  // uniform
-    const int64_t nreg = (count - first + grid - 1) / grid * G::JN;  // this workgroup's regions
+    const int64_t nprob = (count - first + grid - 1) / grid;  // this workgroup's problems
+    // Per-lane byte offsets live in one VGPR each; everything wave-uniform (a region's rows,
+    // a piece's K position, an output row) goes to the instructions' scalar offset or
+    // immediate fields, so the unrolled columns add no per-column address registers.
     const uint32_t a_off = (uint32_t)((row * args.lda + 2 * grp) * 8);
     const uint32_t c_off = (uint32_t)(((wave * 16 + 4 * grp) * args.ldc + r) * 8);
-    const uint32_t c_row = (uint32_t)(args.ldc * 8);
+    const int c_row = (int)(args.ldc * 8);
     // the wave's share: K step sh, pieces si and si + 1 (k = 64 sh + 8 (si + q) + 2 grp + {0, 1})
     const int sh = (2 * wave) >> 3, si = (2 * wave) & 7;
-    const uint32_t b_lane = (uint32_t)((r * args.ldb + 64 * sh + 8 * si + 2 * grp) * 8);
-    const uint32_t b_rows16 = (uint32_t)(16 * args.ldb * 8);
-    auto prob_of = [&](int64_t reg) { return first + reg / G::JN * grid; };
+    const uint32_t b_lane = (uint32_t)((r * args.ldb + 2 * grp) * 8);
+    const int b_share = (64 * sh + 8 * si) * 8;
+    const int b_rows16 = (int)(16 * args.ldb * 8);
 
+    // Buffer descriptors of problem q's operands; with `live` false a descriptor has no
+    // range and its loads return zeros without touching memory.
+    auto rsrc = [&](const double* base, int64_t q, int64_t stride, bool live) {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base + q * stride), (short)0,
+                                                 live ? 0x7FFFFFF0 : 0, 0x00020000);
+    };
     u4_t xa[G::KS][8];
-    auto load_a = [&](int64_t q, bool live) {
-        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-            const_cast<double*>(args.A + q * args.sa), (short)0, live ? 0x7FFFFFF0 : 0, 0x00020000);
-#pragma unroll
-        for (int h = 0; h < G::KS; ++h)
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-                xa[h][i] = __builtin_bit_cast(u4_t, __builtin_amdgcn_raw_buffer_load_b128(
-                                                        rs, (int)(a_off + 8 * (64 * h + 8 * i)), 0, 0));
+    auto load_a_piece = [&](__amdgpu_buffer_rsrc_t rs, auto pc_tag) {
+        constexpr int pc = decltype(pc_tag)::value, h = pc >> 3, i = pc & 7;
+        xa[h][i] = __builtin_bit_cast(
+            u4_t, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)a_off, 8 * (64 * h + 8 * i), 0));
     };
-    // this wave's share of region reg into raw slot reg mod D (a rangeless descriptor past
-    // the last region: no memory traffic)
-    auto dma = [&](int64_t reg) {
-        const bool live = reg < nreg;
-        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-            const_cast<double*>(args.B + (live ? prob_of(reg) : first) * args.sb), (short)0,
-            live ? 0x7FFFFFF0 : 0, 0x00020000);
-        const uint32_t off = b_lane + (uint32_t)(reg % G::JN) * b_rows16;
-        unsigned char* dst = smem + St::RAW_OFF + (int)(reg % St::D) * G::REGION + 2 * wave * G::FRAG;
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                rs, (__attribute__((address_space(3))) void*)(dst + q * G::FRAG), 16,
-                (int)(off + 64 * q), 0, 0, 0);
+    // Region t sits in raw slot t mod D.  With kb = (JN k) mod D the slot of region JN k
+    // (wave-uniform; always 0 when JN is a multiple of D), region JN k + cc of problem k
+    // (cc >= JN: region cc - JN of problem k + 1) takes slot (kb + cc) mod D.
+    auto raw_slot = [&](int kb, int cc) {
+        return (G::JN % St::D == 0 ? cc : kb + cc) % St::D;
     };
-    auto raw_share = [&](int64_t reg, u4_t (&x)[2]) {
+    // this wave's share of region JN k + cc into its raw slot
+    auto dma = [&](int64_t k, int64_t p, int kb, auto cc_tag) {
+        constexpr int cc = decltype(cc_tag)::value, dk = cc / G::JN, jn = cc % G::JN;
+        const bool live = k + dk < nprob;
+        const __amdgpu_buffer_rsrc_t rs = rsrc(args.B, live ? p + dk * grid : first, args.sb, live);
+        unsigned char* dst = smem + St::RAW_OFF + raw_slot(kb, cc) * G::REGION + 2 * wave * G::FRAG;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)dst, 16,
+                                                 (int)b_lane, jn * b_rows16 + b_share, 0, 0);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)(dst + G::FRAG),
+                                                 16, (int)b_lane, jn * b_rows16 + b_share + 64, 0, 0);
+    };
+    auto raw_share = [&](int kb, auto cc_tag, u4_t (&x)[2]) {
+        constexpr int cc = decltype(cc_tag)::value;
         const unsigned char* src =
-            smem + St::RAW_OFF + (int)(reg % St::D) * G::REGION + 2 * wave * G::FRAG + 16 * lane;
+            smem + St::RAW_OFF + raw_slot(kb, cc) * G::REGION + 2 * wave * G::FRAG + 16 * lane;
         x[0] = load_raw<16>(src);
         x[1] = load_raw<16>(src + G::FRAG);
     };
     // partial row maxima of the share (as frexp exponents: the exponent of a maximum is
-    // the maximum of the exponents)
-    auto pmax_write = [&](int64_t reg, const u4_t (&x)[2]) {
+    // the maximum of the exponents), into buffer `buf`
+    auto pmax_write = [&](int buf, const u4_t (&x)[2]) {
         const d2_t x0 = bitcast<d2_t>(x[0]), x1 = bitcast<d2_t>(x[1]);
         double m = __builtin_fmax(__builtin_fmax(__builtin_fabs(x0[0]), __builtin_fabs(x0[1])),
                                   __builtin_fmax(__builtin_fabs(x1[0]), __builtin_fabs(x1[1])));
         m = __builtin_fmax(m, __shfl_xor(m, 16, 64));
         m = __builtin_fmax(m, __shfl_xor(m, 32, 64));
-        pmax[((int)(reg & 1) * G::WAVES + wave) * 16 + r] = __builtin_amdgcn_frexp_exp(m);
+        pmax[(buf * G::WAVES + wave) * 16 + r] = __builtin_amdgcn_frexp_exp(m);
     };
     // digits of the share into plane slot reg mod 2: dword si / 2 of every lane's
     // fragment of plane p, K step sh
@@ -473,16 +516,13 @@ split_i8_stream_kernel(SplitArgs args, int64_t count) {
         int E = pm[0];
 #pragma unroll
         for (int w = 1; w < G::WAVES; ++w) E = max(E, pm[w * 16]);
-        if (wave == 0 && grp == 0) expb[(int)(reg % 3) * 16 + r] = E;
+        if (wave == 0 && grp == 0) expb[(int)(reg & 1) * 16 + r] = E;
         d2_t x0 = bitcast<d2_t>(x[0]), x1 = bitcast<d2_t>(x[1]);
-        int sc = 0;
-        if (__builtin_amdgcn_ballot_w64(E < kTinyRowExp) != 0) {  // rare: tiny rows
-            sc = E < kTinyRowExp ? 1000 : 0;
-            x0[0] = __builtin_amdgcn_ldexp(x0[0], sc);
-            x0[1] = __builtin_amdgcn_ldexp(x0[1], sc);
-            x1[0] = __builtin_amdgcn_ldexp(x1[0], sc);
-            x1[1] = __builtin_amdgcn_ldexp(x1[1], sc);
-        }
+        const int sc = E < kTinyRowExp ? 1000 : 0;  // tiny rows: scaled by 2^1000
+        x0[0] = __builtin_amdgcn_ldexp(x0[0], sc);
+        x0[1] = __builtin_amdgcn_ldexp(x0[1], sc);
+        x1[0] = __builtin_amdgcn_ldexp(x1[0], sc);
+        x1[1] = __builtin_amdgcn_ldexp(x1[1], sc);
         const uint32_t c_e = (uint32_t)(1023 + E + sc);
         uint32_t out[8];
         transpose4(digit_word(x0[0], c_e), digit_word(x0[1], c_e), digit_word(x1[0], c_e),
@@ -496,27 +536,37 @@ split_i8_stream_kernel(SplitArgs args, int64_t count) {
 
     // ---- prologue: regions 0 .. D-1 and the first A rows in flight; maxima of regions
     //      0 and 1, digits of region 0 ----
-    for (int d = 0; d < St::D; ++d) dma(d);
-    load_a(first, true);
+    static_assert(G::JN % 2 == 0, "plane and maxima buffers fixed by the column index");
+    unroll_seq([&](auto d) { dma((int64_t)0, first, 0, d); }, std::make_integer_sequence<int, St::D>{});
+    {
+        const __amdgpu_buffer_rsrc_t rs = rsrc(args.A, first, args.sa, true);
+        unroll_seq([&](auto pc) { load_a_piece(rs, pc); }, std::make_integer_sequence<int, St::PIECES>{});
+    }
     __builtin_amdgcn_s_waitcnt(vmcnt_only(0));
     {
         u4_t x[2];
-        raw_share(0, x);
+        raw_share(0, std::integral_constant<int, 0>{}, x);
         pmax_write(0, x);
-        raw_share(1, x);
+        raw_share(0, std::integral_constant<int, 1>{}, x);
         pmax_write(1, x);
         lds_barrier();
-        raw_share(0, x);
+        raw_share(0, std::integral_constant<int, 0>{}, x);
         slice_share(0, x);
         lds_barrier();
     }
 
     u4_t ad[G::S][G::KS];
     int erow[4];
-    for (int64_t t = 0; t < nreg; ++t) {
-        const int c = (int)(t % G::JN);
-        const int64_t p = prob_of(t);
-        if (c == 0) {
+    int kb = 0;  // raw slot of the problem's region 0
+    for (int64_t k = 0; k < nprob; ++k, kb = (kb + G::JN) % St::D) {
+        const int64_t p = first + k * grid;
+        const bool more = k + 1 < nprob;
+        const __amdgpu_buffer_rsrc_t crs = rsrc(args.C, p, args.sc, true);
+        // column block c of problem k (region t = JN k + c), unrolled over c: the plane and
+        // maxima buffers (JN is even), the A pieces of a column and its wait count are fixed
+        auto column_block = [&](auto c_tag) {
+        constexpr int c = decltype(c_tag)::value;
+        if constexpr (c == 0) {
             // ---- a new problem: A exponents and digits into registers ----
             const int ea = row_exponent<G::KS>(xa);
             expa[row] = ea;
@@ -531,120 +581,142 @@ split_i8_stream_kernel(SplitArgs args, int64_t count) {
             }
 #pragma unroll
             for (int v = 0; v < 4; ++v) erow[v] = expa[wave * 16 + 4 * grp + v];  // own rows
-            const bool more = p + grid < count;
-            load_a(more ? p + grid : p, more);
         }
         // ---- B pipeline: maxima of region t + 2, digits of region t + 1 ----
-        // this wave's loads of region t + 2 (issued two columns ago) have landed: in
-        // order, only the later ones may be in flight -- per column 2 region loads and 4
-        // stores, plus the A loads issued at a problem's first column
-        if (args.beta != 0.0) {
+        // this wave's loads of region t + 2 (issued two columns ago) have landed
+        if constexpr (BETA)
             __builtin_amdgcn_s_waitcnt(vmcnt_only(0));  // C loads change the counts
-        } else if (c <= 1) {
-            __builtin_amdgcn_s_waitcnt(vmcnt_only(10 + 8 * G::KS));
-        } else {
-            __builtin_amdgcn_s_waitcnt(vmcnt_only(10));
+        else
+            __builtin_amdgcn_s_waitcnt(vmcnt_only(St::region_wait(c)));
+        dma(k, p, kb, std::integral_constant<int, c + St::D>{});  // into the slot region t left
+        // this column's share of the next problem's A pieces (the registers of problem k's
+        // A were sliced at column 0)
+        if constexpr (St::a_count(c) > 0) {
+            const __amdgpu_buffer_rsrc_t rs = rsrc(args.A, more ? p + grid : p, args.sa, more);
+            unroll_seq([&](auto j) { load_a_piece(rs, std::integral_constant<int, St::a_first(c) + j>{}); },
+                       std::make_integer_sequence<int, St::a_count(c)>{});
         }
-        dma(t + St::D);  // into the slot region t left
         u4_t x2[2], x1[2];
-        raw_share(t + 2, x2);
-        raw_share(t + 1, x1);
+        raw_share(kb, std::integral_constant<int, c + 2>{}, x2);
+        raw_share(kb, std::integral_constant<int, c + 1>{}, x1);
         // row exponents of region t + 1: the largest of the waves' partial maxima
         int E;
         {
-            const int* pm = pmax + (int)((t + 1) & 1) * G::WAVES * 16 + r;
+            const int* pm = pmax + ((c + 1) & 1) * G::WAVES * 16 + r;
             E = pm[0];
 #pragma unroll
             for (int w = 1; w < G::WAVES; ++w) E = max(E, pm[w * 16]);
         }
-        expb[(int)((t + 1) % 3) * 16 + r] = E;  // every wave and lane group: the same value
-        const int ecol = expb[(int)(t % 3) * 16 + r];
-        const unsigned char* const bsrc = smem + St::PLN_OFF + (int)(t & 1) * G::REGION + 16 * lane;
+        // every wave and lane group writes the same value; written at column t, read at
+        // column t + 1, so two buffers alternate across the barrier between them
+        expb[((c + 1) & 1) * 16 + r] = E;
+        const int ecol = expb[(c & 1) * 16 + r];
+        const unsigned char* const bsrc = smem + St::PLN_OFF + (c & 1) * G::REGION + 16 * lane;
         auto bfrag = [&](int pb, int h) {
             return bitcast<i4_t>(load_raw<16>(bsrc + (pb * G::KS + h) * G::FRAG));
         };
+        // The level products of region t (MFMA) and the VALU work of the column -- the
+        // partial maxima of region t + 2 and the digits of region t + 1, independent of the
+        // products -- in one stretch, interleaved in program order: the products run in
+        // groups, one B fragment (K step h, plane pb) per group, each followed by one step
+        // of the VALU work, so that the VALU issues while the MFMAs run.  Planes go in the
+        // order 0, 7, 1, 6, 2, 5, 3, 4 (8, 1, 7, 2, 6, 3, 5, 4 products): any two
+        // consecutive groups hold at least 9 products, and fragments are read two groups
+        // ahead.  Rows of region t + 1 with E < -960 take the scaled slicing (a scale of 2^0
+        // for the others: four ldexp per lane).
+        static_assert(G::L == G::S && G::S == 8, "eight planes, all levels below 8");
+        constexpr int kOrder[8] = {0, 7, 1, 6, 2, 5, 3, 4};
+        constexpr int NF = 8 * G::KS;
         i4_t T[G::L];
-        // One straight stretch of code: the partial maxima of region t + 2, the digits of
-        // region t + 1 (VALU) and the level products of region t (MFMA), independent of
-        // each other; the scheduling groups at its end interleave them so that the VALU
-        // work issues while the MFMAs run.  TINY: some row of region t + 1 has
-        // E < -960 and takes the scaled slicing.
-        auto column = [&](auto tiny_tag) {
-            constexpr bool TINY = decltype(tiny_tag)::value;
 #pragma unroll
-            for (int l = 0; l < G::L; ++l) T[l] = i4_t{0, 0, 0, 0};
-            i4_t bf = bfrag(0, 0);
-#pragma unroll
-            for (int h = 0; h < G::KS; ++h)
-#pragma unroll
-                for (int pb = 0; pb < G::L && pb < G::S; ++pb) {
-                    const bool last = h + 1 == G::KS && (pb + 1 == G::L || pb + 1 == G::S);
-                    i4_t bn = bf;
-                    if (!last) bn = (pb + 1 < G::L && pb + 1 < G::S) ? bfrag(pb + 1, h) : bfrag(0, h + 1);
-#pragma unroll
-                    for (int pa = 0; pa + pb < G::L && pa < G::S; ++pa)
-                        T[pa + pb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bitcast<i4_t>(ad[pa][h]),
-                                                                           bf, T[pa + pb], 0, 0, 0);
-                    bf = bn;
-                }
-            // the LDS writes of the slicing come after the B fragment reads of the products
-            // (they may not be reordered across them), its arithmetic is free to move up
-            pmax_write(t + 2, x2);
-            {
-                d2_t a0 = bitcast<d2_t>(x1[0]), a1 = bitcast<d2_t>(x1[1]);
-                int sc = 0;
-                if constexpr (TINY) {
-                    sc = E < kTinyRowExp ? 1000 : 0;
-                    a0[0] = __builtin_amdgcn_ldexp(a0[0], sc);
-                    a0[1] = __builtin_amdgcn_ldexp(a0[1], sc);
-                    a1[0] = __builtin_amdgcn_ldexp(a1[0], sc);
-                    a1[1] = __builtin_amdgcn_ldexp(a1[1], sc);
-                }
-                const uint32_t c_e = (uint32_t)(1023 + E + sc);
-                uint32_t out[8];
-                transpose4(digit_word(a0[0], c_e), digit_word(a0[1], c_e), digit_word(a1[0], c_e),
-                           digit_word(a1[1], c_e), out);
-                unsigned char* dst = smem + St::PLN_OFF + (int)((t + 1) & 1) * G::REGION +
-                                     sh * G::FRAG + 16 * lane + 4 * (si >> 1);
+        for (int l = 0; l < G::L; ++l) T[l] = i4_t{0, 0, 0, 0};
+        i4_t fb[3];
+        fb[0] = bfrag(kOrder[0], 0);
+        fb[1] = bfrag(kOrder[1], 0);
+        d2_t a0, a1;
+        uint32_t c_e = 0;
+        uint64_t dw[4];
+        uint32_t out[8];
+        auto valu_step = [&](auto s_tag) {
+            constexpr int s = decltype(s_tag)::value;
+            if constexpr (s == 0) {
+                pmax_write(c & 1, x2);
+            } else if constexpr (s == 1) {
+                a0 = bitcast<d2_t>(x1[0]);
+                a1 = bitcast<d2_t>(x1[1]);
+                const int sc = E < kTinyRowExp ? 1000 : 0;
+                a0[0] = __builtin_amdgcn_ldexp(a0[0], sc);
+                a0[1] = __builtin_amdgcn_ldexp(a0[1], sc);
+                a1[0] = __builtin_amdgcn_ldexp(a1[0], sc);
+                a1[1] = __builtin_amdgcn_ldexp(a1[1], sc);
+                c_e = (uint32_t)(1023 + E + sc);
+            } else if constexpr (s >= 2 && s <= 5) {
+                const double x = s == 2 ? a0[0] : s == 3 ? a0[1] : s == 4 ? a1[0] : a1[1];
+                dw[s - 2] = digit_word(x, c_e);
+            } else if constexpr (s == 6) {
+                transpose4(dw[0], dw[1], dw[2], dw[3], out);
+            } else if constexpr (s == 7) {
+                unsigned char* dst = smem + St::PLN_OFF + ((c + 1) & 1) * G::REGION + sh * G::FRAG +
+                                     16 * lane + 4 * (si >> 1);
 #pragma unroll
                 for (int b = 0; b < 8; ++b)
                     *reinterpret_cast<uint32_t*>(dst + (7 - b) * G::KS * G::FRAG) = out[b];
             }
-            // one MFMA, then two VALU instructions, while the VALU work lasts
-#pragma unroll
-            for (int q = 0; q < G::KS * G::products(); ++q) {
-                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
-            }
         };
-        if (__builtin_amdgcn_ballot_w64(E < kTinyRowExp) != 0)
-            column(std::true_type{});
-        else
-            column(std::false_type{});
-        unsigned char* const Cp = reinterpret_cast<unsigned char*>(args.C + p * args.sc);
+        unroll_seq(
+            [&](auto f_tag) {
+                constexpr int f = decltype(f_tag)::value, h = f / 8, pb = kOrder[f % 8];
+                if constexpr (f + 2 < NF) fb[(f + 2) % 3] = bfrag(kOrder[(f + 2) % 8], (f + 2) / 8);
+#pragma unroll
+                for (int pa = 0; pa + pb < G::L; ++pa)
+                    T[pa + pb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bitcast<i4_t>(ad[pa][h]), fb[f % 3],
+                                                                       T[pa + pb], 0, 0, 0);
+                // the VALU steps after every KS-th group: eight steps per column
+                if constexpr (f % G::KS == 0) valu_step(std::integral_constant<int, f / G::KS>{});
+                __builtin_amdgcn_sched_barrier(0);
+            },
+            std::make_integer_sequence<int, NF>{});
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             double acc = (double)T[G::L - 1][v];
 #pragma unroll
             for (int l = G::L - 2; l >= 0; --l) acc = __builtin_fma(acc, 1.0 / 256.0, (double)T[l][v]);
-            double* dst = reinterpret_cast<double*>(Cp + c_off + v * c_row + c * 128);
+            const int so = v * c_row + c * 128;
             double val = __builtin_amdgcn_ldexp(acc, erow[v] + ecol - 12) * args.alpha;
-            if (args.beta != 0.0) val += args.beta * (*dst);
-            *dst = val;
+            if constexpr (BETA)
+                val += args.beta * __builtin_bit_cast(
+                                       double, __builtin_amdgcn_raw_buffer_load_b64(crs, (int)c_off, so, 0));
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2_t, val), crs, (int)c_off, so, 0);
         }
         lds_barrier();
+        };
+        unroll_seq(column_block, std::make_integer_sequence<int, G::JN>{});
     }
+}
+
+// Raw regions of B held in LDS by the streaming form: region loads run D - 2 columns
+// ahead of their row maxima.
+constexpr int kI8StreamDepth = 4;
+
+// BETA: C is read (beta != 0); a separate instance, so that the columns of the other one
+// carry no branch
+template <class G, bool BETA>
+hipError_t launch_split_i8_stream_form(const SplitArgs& a, int64_t count, hipStream_t s) {
+    constexpr int D = kI8StreamDepth;
+    static std::atomic<int> cache[kMaxCachedDevices];
+    int slots = 0;
+    const hipError_t e = resident_workgroups(split_i8_stream_kernel<G, D, BETA>, G::NT, cache, &slots);
+    if (e != hipSuccess) return e;
+    const int64_t grid = std::min<int64_t>(count, (int64_t)slots);
+    hipLaunchKernelGGL((split_i8_stream_kernel<G, D, BETA>), dim3((unsigned)grid), dim3(G::NT), 0, s, a,
+                       count);
+    return hipGetLastError();
 }
 
 template <class G>
 hipError_t launch_split_i8_stream(const SplitArgs& a, int64_t count, hipStream_t s) {
-    static std::atomic<int> cache[kMaxCachedDevices];
-    int slots = 0;
-    const hipError_t e = resident_workgroups(split_i8_stream_kernel<G>, G::NT, cache, &slots);
-    if (e != hipSuccess) return e;
-    const int64_t grid = std::min<int64_t>(count, (int64_t)slots);
-    hipLaunchKernelGGL((split_i8_stream_kernel<G>), dim3((unsigned)grid), dim3(G::NT), 0, s, a, count);
-    return hipGetLastError();
+    return a.beta != 0.0 ? launch_split_i8_stream_form<G, true>(a, count, s)
+                         : launch_split_i8_stream_form<G, false>(a, count, s);
 }
 
 template <class G, bool REPEAT>
