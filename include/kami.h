@@ -202,6 +202,22 @@ kami_status kami_batched_execute(kami_batched_plan* plan, void* stream);
 kami_status kami_batched_plan_info(const kami_batched_plan* plan, kami_plan_info* out);
 kami_status kami_batched_plan_destroy(kami_batched_plan* plan);
 
+/* One batched GEMM without a plan and without allocating: no counterpart in the reference;
+ * for callers that cannot allocate memory (code under stream capture, serving loops).  The
+ * problems, precision and options mean what they mean for kami_batched_plan_create and
+ * the result equals that of a plan created with the same arguments and executed once;
+ * autotune and use_graph must be 0 (KAMI_EINVAL otherwise: both need memory of their
+ * own), tile variants come from the tuning cache or the priors.  The work queue and its
+ * counter live in a fixed device array of 2 MiB per device (held until the process ends):
+ * at most 32768 queue entries (output tiles of 64 x 64, 32 x 32 or 128 x 64 by size
+ * bucket), KAMI_ENOSPC beyond, checked before any device call.  The entries are built on
+ * the host in static arrays and copied to the device on `stream`; the call allocates no
+ * device or host memory.  Every call on a device shares that device's array: calls that
+ * overlap on one device (two streams, or two host threads whose launches interleave) race
+ * on the queue, so serialise them. */
+kami_status kami_batched_gemm_static(const kami_problem* probs, int64_t count, int32_t precision,
+                                     const kami_batched_opts* opts, void* stream);
+
 /* ---------------------------------------------------------------- block SpGEMM */
 /* Workspace for symbolic/fill of C = A * B, A with mb block rows, B with nb block cols. */
 kami_status kami_bsr_spgemm_workspace_size(int32_t mb, int32_t nb, size_t* bytes);

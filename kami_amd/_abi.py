@@ -92,6 +92,8 @@ _PROTOTYPES = {
     "kami_batched_execute": (c_int32, [c_void_p, c_void_p]),
     "kami_batched_plan_info": (c_int32, [c_void_p, ctypes.POINTER(PlanInfo)]),
     "kami_batched_plan_destroy": (c_int32, [c_void_p]),
+    "kami_batched_gemm_static": (c_int32, [ctypes.POINTER(Problem), c_int64, c_int32,
+                                           ctypes.POINTER(BatchedOpts), c_void_p]),
     "kami_bsr_spgemm_workspace_size": (c_int32, [c_int32, c_int32, ctypes.POINTER(c_size_t)]),
     "kami_bsr_spgemm_symbolic": (c_int32, [ctypes.POINTER(Bsr), ctypes.POINTER(Bsr), c_void_p,
                                            ctypes.POINTER(c_int64), c_void_p, c_size_t, c_void_p]),
@@ -135,6 +137,15 @@ def lib():
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it with `make -C kami_amd/csrc` "
                 "(or __graft_entry__.build()); there is no fallback implementation")
+        # One HIP runtime per process: torch's bundled libamdhip64.so carries the soname
+        # libamdhip64.so.7 that libkami.so needs, so once torch is loaded libkami binds to
+        # that same runtime.  Loaded the other way round, torch would load its own copy
+        # by file name, and of two runtimes in one process the second to start finds no
+        # device (hipErrorNoDevice).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         handle = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _PROTOTYPES.items():
             fn = getattr(handle, name)

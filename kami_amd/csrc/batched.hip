@@ -20,6 +20,7 @@
 // M, N and K must be multiples of 16.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -657,23 +658,38 @@ static hipError_t persistent_grid(int prec, bool kn, int* grid) {
 // once, so tiles that share rows of A or columns of B run on one CU, back to back, and
 // find those operands in its L2 instead of fetching them again (ordering by tile cost
 // scattered them over all XCDs and read A and B 1.8 times over).
-static void build_items(const std::vector<kami_problem>& probs, const std::vector<int>& bucket,
-                        const int* variant_of_bucket, int ei, int eo, bool b_kn,
-                        std::vector<WorkItem>& items) {
-    std::vector<int64_t> order(probs.size());
-    for (size_t p = 0; p < probs.size(); ++p) order[p] = (int64_t)p;
-    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+// Queue entries of `count` problems (bucket[p] their size bucket), written to out[];
+// `order` is room for `count` indices.  Equal-cost problems keep their input order
+// (an index tie-break instead of a stable sort, which may allocate).
+static int64_t count_items(const kami_problem* probs, int64_t count, const int* bucket,
+                           const int* variant_of_bucket) {
+    int64_t n = 0;
+    for (int64_t p = 0; p < count; ++p) {
+        const int v = variant_of_bucket[bucket[p]];
+        n += (int64_t)((probs[p].m + kVariantTM[v] - 1) / kVariantTM[v]) *
+             ((probs[p].n + kVariantTN[v] - 1) / kVariantTN[v]);
+    }
+    return n;
+}
+
+static void build_items(const kami_problem* probs, int64_t count, const int* bucket,
+                        const int* variant_of_bucket, int ei, int eo, bool b_kn, int64_t* order,
+                        WorkItem* out) {
+    for (int64_t p = 0; p < count; ++p) order[p] = p;
+    std::sort(order, order + count, [&](int64_t a, int64_t b) {
         const kami_problem &P = probs[a], &Q = probs[b];
-        return (int64_t)P.m * P.n * P.k > (int64_t)Q.m * Q.n * Q.k;
+        const int64_t ca = (int64_t)P.m * P.n * P.k, cb = (int64_t)Q.m * Q.n * Q.k;
+        return ca != cb ? ca > cb : a < b;
     });
-    items.clear();
-    for (int64_t p : order) {
+    int64_t i = 0;
+    for (int64_t q = 0; q < count; ++q) {
+        const int64_t p = order[q];
         const kami_problem& P = probs[p];
         const int v = variant_of_bucket[bucket[p]];
         const int tm = kVariantTM[v], tn = kVariantTN[v];
         for (int r = 0; r < P.m; r += tm)
             for (int c = 0; c < P.n; c += tn) {
-                WorkItem w;
+                WorkItem& w = out[i++];
                 w.A = static_cast<const char*>(P.A) + (int64_t)r * P.lda * ei;
                 w.B = static_cast<const char*>(P.B) + (b_kn ? (int64_t)c : (int64_t)c * P.ldb) * ei;
                 w.C = static_cast<char*>(P.C) + ((int64_t)r * P.ldc + c) * eo;
@@ -684,8 +700,49 @@ static void build_items(const std::vector<kami_problem>& probs, const std::vecto
                 w.nlim = P.n - c;
                 w.k = P.k;
                 w.variant = v;
-                items.push_back(w);
             }
+    }
+}
+
+static void build_items(const std::vector<kami_problem>& probs, const std::vector<int>& bucket,
+                        const int* variant_of_bucket, int ei, int eo, bool b_kn,
+                        std::vector<WorkItem>& items) {
+    const int64_t count = (int64_t)probs.size();
+    std::vector<int64_t> order(probs.size());
+    items.resize((size_t)count_items(probs.data(), count, bucket.data(), variant_of_bucket));
+    build_items(probs.data(), count, bucket.data(), variant_of_bucket, ei, eo, b_kn, order.data(),
+                items.data());
+}
+
+// Validates one problem of a batch and fills in packed leading dimensions.
+static kami_status check_problem(kami_problem& p, int b_layout, int ei, int eo) {
+    if (p.m <= 0 || p.n <= 0 || p.k <= 0 || p.m % 16 || p.n % 16 || p.k % 16) return KAMI_EINVAL;
+    if (!p.A || !p.B || !p.C) return KAMI_EINVAL;
+    if (!p.lda) p.lda = p.k;
+    if (!p.ldb) p.ldb = b_layout == KAMI_B_NK ? p.k : p.n;
+    if (!p.ldc) p.ldc = p.n;
+    if (p.lda < p.k || p.ldb < (b_layout == KAMI_B_NK ? p.k : p.n) || p.ldc < p.n)
+        return KAMI_EINVAL;
+    if ((p.lda * ei) % 16 || (p.ldb * ei) % 16 || ((uintptr_t)p.A % 16) || ((uintptr_t)p.B % 16) ||
+        ((uintptr_t)p.C % eo))
+        return KAMI_EINVAL;
+    // Operand pieces are addressed by 32-bit byte offsets from the tile's origin
+    // (stream_load): the farthest piece of any tile must end below kBufferRange.
+    const int64_t a_span = ((int64_t)(std::min(p.m, 128) - 1) * p.lda + p.k) * ei;
+    const int64_t b_span = b_layout == KAMI_B_NK
+                               ? ((int64_t)(std::min(p.n, 64) - 1) * p.ldb + p.k) * ei
+                               : ((int64_t)(p.k - 1) * p.ldb + std::min(p.n, 64)) * ei;
+    if (a_span > kBufferRange || b_span > kBufferRange) return KAMI_ENOTSUP;
+    return KAMI_OK;
+}
+
+// Tile variant of every size bucket without timing: the tuning cache's choice, else the
+// prior.
+static void untimed_variants(int precision, int (&variant_of_bucket)[125]) {
+    for (int b = 0; b < 125; ++b) {
+        const int tuned = batched_tuned_variant(precision, b);
+        variant_of_bucket[b] =
+            tuned >= 0 && tuned < variants_of(precision) ? tuned : prior_variant(precision, b);
     }
 }
 
@@ -699,6 +756,44 @@ static kami_status upload(const void* host, size_t bytes, void** dev) {
         e = hipMemcpy(*dev, host, bytes, hipMemcpyHostToDevice);
         if (e != hipSuccess) return hip_fail(e);
     }
+    return KAMI_OK;
+}
+
+// ---- one-shot execution without allocation ----
+// The queue and its counter of kami_batched_gemm_static live in `scratch`, a file-scope
+// device array (each device holds its own copy until the process ends): the counter in its
+// first 256 bytes, then up to kStaticItems queue entries.  Host code reaches a device's
+// copy only through the address hipGetSymbolAddress returns with that device current (the
+// array's name is a host-side stand-in, not a device address).  The host side stages the
+// problems, their order and the entries in static arrays, so the call makes no allocation
+// of device or host memory; a mutex keeps two host threads from sharing the staging.
+constexpr int64_t kStaticItems = 32768;  // 2 MiB of queue entries
+constexpr size_t kStaticCounterBytes = 256;
+static __device__ __align__(256) unsigned char scratch[kStaticCounterBytes + kStaticItems * sizeof(WorkItem)];
+
+struct StaticStaging {
+    std::mutex mu;
+    unsigned char* dev_ptr[kMaxCachedDevices] = {};  // scratch of each device
+    kami_problem probs[kStaticItems];  // a problem holds at least one entry
+    int bucket[kStaticItems];
+    int64_t order[kStaticItems];
+    WorkItem items[kStaticItems];
+};
+static StaticStaging g_static;
+
+// scratch of the current device (caller holds g_static.mu)
+static kami_status static_scratch(unsigned char** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e);
+    if (dev < 0 || dev >= kMaxCachedDevices) return KAMI_ENOTSUP;
+    if (!g_static.dev_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e);
+        g_static.dev_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_static.dev_ptr[dev];
     return KAMI_OK;
 }
 
@@ -733,23 +828,8 @@ kami_status kami_batched_plan_create(const kami_problem* probs, int64_t count, i
     std::vector<kami_problem> hp(probs, probs + count);
     double flops = 0;
     for (auto& p : hp) {
-        if (p.m <= 0 || p.n <= 0 || p.k <= 0 || p.m % 16 || p.n % 16 || p.k % 16) return KAMI_EINVAL;
-        if (!p.A || !p.B || !p.C) return KAMI_EINVAL;
-        if (!p.lda) p.lda = p.k;
-        if (!p.ldb) p.ldb = o.b_layout == KAMI_B_NK ? p.k : p.n;
-        if (!p.ldc) p.ldc = p.n;
-        if (p.lda < p.k || p.ldb < (o.b_layout == KAMI_B_NK ? p.k : p.n) || p.ldc < p.n)
-            return KAMI_EINVAL;
-        if ((p.lda * ei) % 16 || (p.ldb * ei) % 16 || ((uintptr_t)p.A % 16) ||
-            ((uintptr_t)p.B % 16) || ((uintptr_t)p.C % eo))
-            return KAMI_EINVAL;
-        // Operand pieces are addressed by 32-bit byte offsets from the tile's origin
-        // (stream_load): the farthest piece of any tile must end below kBufferRange.
-        const int64_t a_span = ((int64_t)(std::min(p.m, 128) - 1) * p.lda + p.k) * ei;
-        const int64_t b_span = o.b_layout == KAMI_B_NK
-                                   ? ((int64_t)(std::min(p.n, 64) - 1) * p.ldb + p.k) * ei
-                                   : ((int64_t)(p.k - 1) * p.ldb + std::min(p.n, 64)) * ei;
-        if (a_span > kBufferRange || b_span > kBufferRange) return KAMI_ENOTSUP;
+        const kami_status pst = check_problem(p, o.b_layout, ei, eo);
+        if (pst != KAMI_OK) return pst;
         flops += 2.0 * p.m * p.n * (double)p.k;
     }
     kami_status st = device_check();
@@ -773,11 +853,7 @@ kami_status kami_batched_plan_create(const kami_problem* probs, int64_t count, i
         bucket[i] = bucket_of(hp[i]);
         used[bucket[i]] = true;
     }
-    for (int b = 0; b < 125; ++b) {
-        const int tuned = batched_tuned_variant(precision, b);
-        variant_of_bucket[b] = tuned >= 0 && tuned < variants_of(precision) ? tuned
-                                                                            : prior_variant(precision, b);
-    }
+    untimed_variants(precision, variant_of_bucket);
 
     hipError_t e = persistent_grid(precision, plan->b_kn, &plan->grid);
     if (e != hipSuccess) {
@@ -920,6 +996,59 @@ kami_status kami_batched_plan_info(const kami_batched_plan* plan, kami_plan_info
 
 kami_status kami_batched_plan_destroy(kami_batched_plan* plan) {
     free_plan(plan);
+    return KAMI_OK;
+}
+
+kami_status kami_batched_gemm_static(const kami_problem* probs, int64_t count, int32_t precision,
+                                     const kami_batched_opts* opts, void* stream) {
+    if (!probs || count <= 0) return KAMI_EINVAL;
+    if (precision != KAMI_PREC_F64 && precision != KAMI_PREC_F16_F32 &&
+        precision != KAMI_PREC_BF16_F32)
+        return KAMI_ENOTSUP;
+    kami_batched_opts o;
+    std::memset(&o, 0, sizeof(o));
+    o.alpha = 1.0;
+    if (opts) o = *opts;
+    if (o.b_layout != KAMI_B_NK && o.b_layout != KAMI_B_KN) return KAMI_EINVAL;
+    if (o.autotune || o.use_graph) return KAMI_EINVAL;  // both need memory of their own
+    if (count > kStaticItems) return KAMI_ENOSPC;
+    const int ei = elt_in(precision), eo = elt_out(precision);
+    const bool b_kn = o.b_layout == KAMI_B_KN;
+
+    std::lock_guard<std::mutex> lock(g_static.mu);
+    int variant_of_bucket[125];
+    untimed_variants(precision, variant_of_bucket);
+    for (int64_t i = 0; i < count; ++i) {
+        kami_problem& p = g_static.probs[i];
+        p = probs[i];
+        const kami_status pst = check_problem(p, o.b_layout, ei, eo);
+        if (pst != KAMI_OK) return pst;
+        g_static.bucket[i] = bucket_of(p);
+    }
+    const int64_t n_items = count_items(g_static.probs, count, g_static.bucket, variant_of_bucket);
+    if (n_items > kStaticItems) return KAMI_ENOSPC;
+    kami_status st = device_check();
+    if (st != KAMI_OK) return st;
+    unsigned char* base = nullptr;
+    st = static_scratch(&base);
+    if (st != KAMI_OK) return st;
+    int grid = 0;
+    hipError_t e = persistent_grid(precision, b_kn, &grid);
+    if (e != hipSuccess) return hip_fail(e);
+    build_items(g_static.probs, count, g_static.bucket, variant_of_bucket, ei, eo, b_kn,
+                g_static.order, g_static.items);
+    hipStream_t s = (hipStream_t)stream;
+    // From pageable memory the copy has consumed the staging array when the call returns,
+    // so the next call may refill it.
+    WorkItem* d_items = reinterpret_cast<WorkItem*>(base + kStaticCounterBytes);
+    e = hipMemcpyAsync(d_items, g_static.items, (size_t)n_items * sizeof(WorkItem),
+                       hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(e);
+    // launch_batched resets the counter with a kernel before the persistent launch
+    BatchArgs a{d_items, reinterpret_cast<unsigned int*>(base), n_items, o.alpha, o.beta,
+                (int32_t)b_kn, queue_shift(n_items, grid)};
+    e = launch_batched(precision, a, grid, s);
+    if (e != hipSuccess) return hip_fail(e);
     return KAMI_OK;
 }
 
