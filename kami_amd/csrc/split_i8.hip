@@ -69,6 +69,9 @@ struct I8Geom {
     static_assert(L >= 1 && L <= 2 * S - 1, "levels");
     // at most S products of two digits (|d| <= 128) per k and level
     static_assert((double)S * BK * 16384.0 < 2147483648.0, "level sums exact in int32");
+    // repetitions of the products whose summed level sums stay exact in int32
+    static constexpr int ACC_ITERS = (int)(2147483647LL / ((long long)S * BK * 16384));
+    static_assert(ACC_ITERS >= 1, "one repetition fits");
     static constexpr int products() {
         int n = 0;
         for (int p = 0; p < S; ++p)
@@ -207,7 +210,6 @@ split_i8_kernel(SplitArgs args, int64_t count) {
     const int row = wave * 16 + r;  // operand row this lane slices, of A and of B
     const int64_t grid = gridDim.x;
     const uint32_t a_off = (uint32_t)((row * args.lda + 2 * grp) * 8);
-    const uint32_t b_off = (uint32_t)((row * args.ldb + 2 * grp) * 8);
     // byte offset of the lane's output (row 16 wave + 4 grp + v, column 16 jn + r)
     const uint32_t c_off = (uint32_t)(((wave * 16 + 4 * grp) * args.ldc + r) * 8);
     const uint32_t c_row = (uint32_t)(args.ldc * 8);
@@ -323,28 +325,36 @@ split_i8_kernel(SplitArgs args, int64_t count) {
             auto bfrag = [&](int pb, int h) {
                 return bitcast<i4_t>(load_raw<16>(bsrc + (pb * G::KS + h) * G::FRAG));
             };
+            // The repetitions accumulate the level sums in int32 (exact while a run of them
+            // stays below 2^31: ACC_ITERS repetitions) and are combined in FP64 once per
+            // run.  B fragments go in the plane order 0, 7, 1, 6, 2, 5, 3, 4 (8, 1, 7, ...
+            // products each), read two ahead through a ring of four; the last two reads of
+            // a repetition fetch the next one's first fragments (the same data).
+            static_assert(G::L == G::S && G::S == 8, "eight planes, all levels below 8");
+            constexpr int kOrder[8] = {0, 7, 1, 6, 2, 5, 3, 4};
+            constexpr int NF = 8 * G::KS;
             double sum[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int it = 0; it < (REPEAT ? args.iters : 1); ++it) {
+            i4_t fb[4];
+            fb[0] = bfrag(kOrder[0], 0);
+            fb[1] = bfrag(kOrder[1], 0);
+            const int iters = REPEAT ? args.iters : 1;
+            for (int it = 0; it < iters;) {
+                const int run = REPEAT ? min(iters - it, G::ACC_ITERS) : 1;
                 i4_t T[G::L];
 #pragma unroll
                 for (int l = 0; l < G::L; ++l) T[l] = i4_t{0, 0, 0, 0};
-                // B fragments one ahead of the products that use them
-                i4_t bf = bfrag(0, 0);
+                for (int rr = 0; rr < run; ++rr) {
 #pragma unroll
-                for (int h = 0; h < G::KS; ++h)
+                    for (int f = 0; f < NF; ++f) {
+                        const int h = f / 8, pb = kOrder[f % 8];
+                        fb[(f + 2) % 4] = bfrag(kOrder[(f + 2) % 8], ((f + 2) % NF) / 8);
 #pragma unroll
-                    for (int pb = 0; pb < G::L && pb < G::S; ++pb) {
-                        const bool last = h + 1 == G::KS && (pb + 1 == G::L || pb + 1 == G::S);
-                        i4_t bn = bf;
-                        if (!last)
-                            bn = (pb + 1 < G::L && pb + 1 < G::S) ? bfrag(pb + 1, h) : bfrag(0, h + 1);
-#pragma unroll
-                        for (int pa = 0; pa + pb < G::L && pa < G::S; ++pa)
+                        for (int pa = 0; pa + pb < G::L; ++pa)
                             T[pa + pb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                                bitcast<i4_t>(ad[pa][h]), bf, T[pa + pb], 0, 0, 0);
-                        bf = bn;
+                                bitcast<i4_t>(ad[pa][h]), fb[f % 4], T[pa + pb], 0, 0, 0);
                         __builtin_amdgcn_sched_barrier(0);
                     }
+                }
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     double acc = (double)T[G::L - 1][v];
@@ -354,6 +364,7 @@ split_i8_kernel(SplitArgs args, int64_t count) {
                     sum[v] += acc;
                     __builtin_amdgcn_sched_barrier(0);  // one output's conversions at a time
                 }
+                it += run;
             }
 #pragma unroll
             for (int v = 0; v < 4; ++v) {

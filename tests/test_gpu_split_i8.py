@@ -136,6 +136,29 @@ def test_resident_repeat_kat_exact(s, iters, dev):
 
 
 @pytest.mark.parametrize("s", SIZES)
+def test_resident_repeat_past_one_int32_run(s, dev):
+    """The repetitions sum the level sums in int32 for at most 2^31 / (8 K 2^14) of them
+    (255 at K = 64, 127 at K = 128), then combine in FP64 and start again: iters past
+    that bound still give iters * A B exactly on integer inputs, and on random inputs
+    (digits over the whole byte range, the largest level sums) the same as one product
+    times iters to within the final rounding."""
+    iters = 300
+    count = 3
+    rng = np.random.default_rng(41 + s)
+    ramp = np.tile(oracle.ramp(s * s).reshape(1, s, s), (count, 1, 1))
+    randr = oracle.randr_mod3(count * s * s).reshape(count, s, s) - 1.0
+    np.testing.assert_array_equal(run(ramp, randr, dev, iters=iters), iters * oracle.gemm_f64(ramp, randr))
+    A = rng.uniform(-1, 1, (count, s, s))
+    B = rng.uniform(-1, 1, (count, s, s))
+    C1 = run(A, B, dev)
+    Cn = run(A, B, dev, iters=iters)
+    exact, absC = oracle.gemm_dot2(A, B)
+    comp, _ = errors(Cn / iters, exact, absC)
+    assert comp <= 1e-13
+    assert np.max(np.abs(Cn / iters - C1) / np.where(absC > 0, absC, 1.0)) <= 1e-15
+
+
+@pytest.mark.parametrize("s", SIZES)
 def test_resident_repeat_log_uniform(s, dev):
     rng = np.random.default_rng(37 + s)
     A = log_uniform(rng, (16, s, s))
