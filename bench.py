@@ -197,6 +197,8 @@ def measured_lds(path, cfg, m, count):
            "bank_conflict_cycles": rec["SQ_LDS_BANK_CONFLICT"], "source": "rocprofv3 --pmc SQ_INSTS_LDS"}
     if rec.get("bytes_per_instruction"):
         out["bytes_per_flop"] = rec["SQ_INSTS_LDS"] * rec["bytes_per_instruction"] / rec["flop_per_launch"]
+    elif rec.get("lds_bytes_per_launch"):  # mixed widths: bytes counted from the kernel's mix
+        out["bytes_per_flop"] = rec["lds_bytes_per_launch"] / rec["flop_per_launch"]
     return out
 
 

@@ -1,8 +1,10 @@
 """Small drivers for profiling one kernel under rocprofv3.
 
     python -m kami_amd.profiling copy --mib 1024         # plain device copy (calibration)
+    python -m kami_amd.profiling add --mib 512           # C = A + B: the w3 traffic mix
     python -m kami_amd.profiling w3 --path native        # FP64 128^3 x 4096 native
     python -m kami_amd.profiling w3 --path split         # same through FP16 slices
+    python -m kami_amd.profiling w3 --path split_i8      # same through int8 digits
     python -m kami_amd.profiling w1 | w3r | w4 --prec f16 | w5 | w6
 
 Each run launches the kernel `--reps` times after one warm-up, so per-dispatch counters
@@ -20,9 +22,9 @@ import numpy as np
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["copy", "w1", "w3", "w3r", "w4", "w5", "w6"])
+    ap.add_argument("what", choices=["copy", "add", "w1", "w3", "w3r", "w4", "w5", "w6"])
     ap.add_argument("--mib", type=int, default=1024)
-    ap.add_argument("--path", choices=["native", "split"], default="native")
+    ap.add_argument("--path", choices=["native", "split", "split_i8"], default="native")
     ap.add_argument("--prec", choices=["f16", "f64"], default="f16")
     ap.add_argument("--reps", type=int, default=3)
     args = ap.parse_args(argv)
@@ -42,6 +44,17 @@ def main(argv=None):
         torch.cuda.synchronize()
         print(f"copy: {args.reps + 1} launches of {args.mib} MiB read + {args.mib} MiB written")
         return
+    if args.what == "add":
+        # two reads and one write of --mib MiB each: the bytes of one w3 launch at 512
+        n = args.mib * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device=dev)
+        b = torch.ones_like(a)
+        c = torch.empty_like(a)
+        for _ in range(args.reps + 1):
+            torch.add(a, b, out=c)
+        torch.cuda.synchronize()
+        print(f"add: {args.reps + 1} launches of 2 x {args.mib} MiB read + {args.mib} MiB written")
+        return
     if args.what == "w3":
         rng = np.random.default_rng(W.SEED + 2)
         m = n = k = 128
@@ -49,7 +62,8 @@ def main(argv=None):
         A = torch.from_numpy(W.log_uniform_signed(rng, (count, m, k))).to(dev)
         B = torch.from_numpy(W.log_uniform_signed(rng, (count, n, k))).to(dev)
         C = torch.empty((count, m, n), dtype=torch.float64, device=dev)
-        prec = K.PREC_F64 if args.path == "native" else K.PREC_F64_SPLIT_F16
+        prec = {"native": K.PREC_F64, "split": K.PREC_F64_SPLIT_F16,
+                "split_i8": K.PREC_F64_SPLIT_I8}[args.path]
         for _ in range(args.reps + 1):
             K.block_gemm(A, B, C, precision=prec)
         torch.cuda.synchronize()

@@ -194,3 +194,52 @@ def test_int8_level_sums_fit_int32_and_the_horner_scale():
     T = [sum(int(da[p, k]) * int(db[l - p, k]) for k in range(5) for p in range(8) if 0 <= l - p < 8)
          for l in range(15)]
     assert sum(t << (8 * (14 - l)) for l, t in enumerate(T)) == exact
+
+
+def _int8_split_model(A, B, levels):
+    """The int8 split's arithmetic in exact integers: per-row exponents over K, 62-bit
+    fixed point truncated toward zero, balanced byte digits, level sums of the digit
+    products for l < levels in int64 (exact), Horner in FP64 and the 2^(e + f - 12) scale."""
+    def digits(X):
+        E = np.frexp(np.max(np.abs(X), axis=-1))[1]
+        y = np.ldexp(np.abs(X), (30 - E)[..., None])
+        hi = np.floor(y)
+        lo = np.floor(np.ldexp(y - hi, 32))
+        V = (hi.astype(np.int64) << 32) + lo.astype(np.int64)  # trunc(|x| 2^(62 - E))
+        V = np.where(X < 0, -V, V)
+        d = []
+        for _ in range(8):
+            r = ((V + 128) & 255) - 128
+            d.append(r)
+            V = (V - r) >> 8
+        assert np.all(V == 0)
+        return E, d[::-1]
+    Ea, da = digits(A)
+    Eb, db = digits(B)
+    T = [np.zeros(A.shape[:-1] + (B.shape[-2],), np.int64) for _ in range(levels)]
+    for p in range(8):
+        for q in range(8):
+            if p + q < levels:
+                T[p + q] += np.matmul(da[p], np.swapaxes(db[q], -1, -2))
+    acc = T[levels - 1].astype(np.float64)
+    for l in range(levels - 2, -1, -1):
+        acc = acc / 256.0 + T[l].astype(np.float64)
+    return np.ldexp(acc, Ea[..., :, None] + Eb[..., None, :] - 12)
+
+
+def test_int8_level_cutoff_meets_the_componentwise_bound_with_eight_levels_only():
+    """On the split-precision workload (sign-random, log10|x| ~ U[-4, 4], 128^3), keeping
+    the levels l < 8 (36 digit products) is within 1e-13 |A||B| of the Dot2 product
+    componentwise; dropping level 7 (28 products) is not."""
+    import oracle
+    rng = np.random.default_rng(0x4B414D49 + 2)
+    shape = (6, 128, 128)
+    A = np.where(rng.random(shape) < 0.5, -1.0, 1.0) * 10.0 ** rng.uniform(-4, 4, shape)
+    B = np.where(rng.random(shape) < 0.5, -1.0, 1.0) * 10.0 ** rng.uniform(-4, 4, shape)
+    exact, absC = oracle.gemm_dot2(A, B)
+    comp = {}
+    for L in (8, 7):
+        d = np.abs(_int8_split_model(A, B, L) - exact)
+        comp[L] = float(np.max(d / np.where(absC > 0, absC, 1.0)))
+    assert comp[8] <= 1e-14
+    assert comp[7] > 1e-13
