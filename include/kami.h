@@ -13,7 +13,7 @@
  *   - Every pointer argument that names matrix data is a device pointer. The library
  *     never frees caller memory and never allocates in kami_block_gemm.
  *   - `stream` is a hipStream_t passed as void* (NULL = the null stream). Every
- *     launch is asynchronous on that stream; the only synchronising call is
+ *     launch is asynchronous on that stream; the only call that waits for the device is
  *     kami_bsr_spgemm_symbolic, which reads the output block count back to the host.
  *   - Errors are returned as kami_status; the library prints nothing.
  *
@@ -84,8 +84,8 @@ typedef struct kami_gemm_desc {
     int32_t b_layout;  /* kami_b_layout */
     int32_t iters;     /* >= 1: the product is accumulated `iters` times before the
                           epilogue, as the reference's NUM_ITER repeat does; 1 = plain GEMM.
-                          Split precision takes iters > 1 at 32^3 and 64^3 (resident form:
-                          digits made once, the slice products and recombination repeated) */
+                          Split precision takes iters > 1 at 32^3 and 64^3 (FP16/BF16) and
+                          64^3 and 128^3 (int8): digits made once, the slice products repeated */
     int64_t lda, ldb, ldc; /* elements; 0 = packed */
     double alpha, beta;    /* C = alpha * (A B) + beta * C */
     int32_t slices;        /* split precision: slices per operand, 0 = default */
@@ -219,14 +219,24 @@ kami_status kami_batched_gemm_static(const kami_problem* probs, int64_t count, i
                                      const kami_batched_opts* opts, void* stream);
 
 /* ---------------------------------------------------------------- block SpGEMM */
-/* Workspace for symbolic/fill of C = A * B, A with mb block rows, B with nb block cols. */
+/* Workspace for symbolic/fill of C = A * B, A with mb block rows, B with nb block cols:
+ * one column bitmap per block row of C, mb * ceil(nb / 32) * 4 bytes. */
 kami_status kami_bsr_spgemm_workspace_size(int32_t mb, int32_t nb, size_t* bytes);
 /* rowptrC (mb+1, device) receives the exclusive scan of per-row output block counts;
- * *nnzbC receives rowptrC[mb] (copied to the host: this call synchronises `stream`).
- * KAMI_ENOTSUP when C would hold 2^31 - 1 blocks or more (int32 row pointers). */
+ * *nnzbC receives rowptrC[mb] (read back on the host: this call waits until the work on
+ * `stream` up to and including its own has stored the total).
+ * KAMI_ENOTSUP when C would hold 2^31 - 1 blocks or more (int32 row pointers).
+ * ws (device) is optional: with at least kami_bsr_spgemm_workspace_size bytes the rows'
+ * column bitmaps are left there for kami_bsr_spgemm_fill, which then only expands them;
+ * pass the same ws, unchanged, to the fill of the same A and B.  The total is written by
+ * the device into a pinned host word that the host waits on (a spin of up to 200 us, then
+ * a blocking wait on the stream); the first call on a host thread allocates that word
+ * (64 bytes, kept until the process ends). */
 kami_status kami_bsr_spgemm_symbolic(const kami_bsr* A, const kami_bsr* B, int32_t* rowptrC,
                                      int64_t* nnzbC, void* ws, size_t ws_bytes, void* stream);
-/* colidxC (nnzbC, device) receives the ascending output block columns of every row. */
+/* colidxC (nnzbC, device) receives the ascending output block columns of every row: from
+ * the bitmaps the symbolic call left in ws when ws holds the workspace size, else
+ * rebuilt from A and B. */
 kami_status kami_bsr_spgemm_fill(const kami_bsr* A, const kami_bsr* B, const int32_t* rowptrC,
                                  int32_t* colidxC, void* ws, size_t ws_bytes, void* stream);
 /* valC (nnzbC * bs * bs, device) = alpha * (A * B) on the structure of rowptrC/colidxC.

@@ -313,9 +313,25 @@ class BsrMatrix:
     def nnzb(self):
         return int(self.colidx.numel())
 
+    def _key(self):
+        """What a successful check and the C descriptor depend on: the arrays' identities
+        and storage, and the block sizes (reassigning or reallocating any of them, or
+        changing a size, makes a new key)."""
+        return (self.mb, self.nb, self.bs, id(self.rowptr), id(self.colidx), id(self.vals),
+                self.rowptr.data_ptr(), self.colidx.data_ptr(), self.vals.data_ptr(),
+                self.colidx.numel())
+
     def check(self, name="matrix"):
         """Raise ValueError unless the arrays match the sizes: rowptr int32 (mb + 1,),
-        colidx int32 (nnzb,), vals (nnzb, bs, bs), all contiguous on one device."""
+        colidx int32 (nnzb,), vals (nnzb, bs, bs), all contiguous on one device.  A
+        passing check is remembered for the same arrays."""
+        key = self._key()
+        if getattr(self, "_checked", None) == key:
+            return
+        self._check(name)
+        self._checked = key
+
+    def _check(self, name):
         import torch
         if self.rowptr.dtype != torch.int32 or tuple(self.rowptr.shape) != (self.mb + 1,):
             raise ValueError(f"{name}: rowptr must be int32 of shape ({self.mb + 1},)")
@@ -330,11 +346,17 @@ class BsrMatrix:
                 raise ValueError(f"{name}: rowptr, colidx and vals must be on one device")
 
     def _c(self):
+        """The C descriptor (kami_bsr), rebuilt only when the arrays change."""
+        key = self._key()
+        cached = getattr(self, "_desc", None)
+        if cached is not None and cached[0] == key:
+            return cached[1]
         b = _abi.Bsr()
         b.mb, b.nb, b.bs, b.nnzb = self.mb, self.nb, self.bs, self.nnzb
         b.rowptr = self.rowptr.data_ptr()
         b.colidx = self.colidx.data_ptr() if self.nnzb else None
         b.vals = self.vals.data_ptr() if self.nnzb else None
+        self._desc = (key, b)
         return b
 
 
@@ -348,23 +370,46 @@ def _check_pair(A: BsrMatrix, B: BsrMatrix):
         raise ValueError("A and B must be on one device")
 
 
-def spgemm_symbolic(A: BsrMatrix, B: BsrMatrix, stream=None):
+def spgemm_workspace(A: BsrMatrix, B: BsrMatrix):
+    """Device workspace for spgemm_symbolic / spgemm_fill of C = A * B: the rows' column
+    bitmaps, which the symbolic call leaves for the fill (uint8 tensor)."""
+    import torch
+    _check_pair(A, B)
+    n = ctypes.c_size_t()
+    check(lib().kami_bsr_spgemm_workspace_size(A.mb, B.nb, ctypes.byref(n)),
+          "kami_bsr_spgemm_workspace_size")
+    return torch.empty(max(int(n.value), 1), dtype=torch.uint8, device=A.rowptr.device)
+
+
+def _workspace_args(ws, device):
+    if ws is None:
+        return None, 0
+    import torch
+    if ws.dtype != torch.uint8 or ws.dim() != 1 or not ws.is_contiguous() or ws.device != device:
+        raise ValueError("workspace must be a contiguous 1-D uint8 tensor on the matrices' device")
+    return ctypes.c_void_p(ws.data_ptr()), ws.numel()
+
+
+def spgemm_symbolic(A: BsrMatrix, B: BsrMatrix, stream=None, workspace=None):
     """Row pointer of C = A * B (exclusive scan of per-row block counts) and nnzb(C).
     Synchronises the stream to read nnzb(C) back, as the reference does
-    (src/sparse/spgemm/2d/block_spgemm_2d_half_mma.cu:808)."""
+    (src/sparse/spgemm/2d/block_spgemm_2d_half_mma.cu:808).  With `workspace`
+    (spgemm_workspace) the rows' column bitmaps stay there for spgemm_fill."""
     import torch
     _check_pair(A, B)
     rowptr = torch.empty(A.mb + 1, dtype=torch.int32, device=A.rowptr.device)
     nnz = ctypes.c_int64()
     a, b = A._c(), B._c()
+    ws, ws_bytes = _workspace_args(workspace, A.rowptr.device)
     check(lib().kami_bsr_spgemm_symbolic(ctypes.byref(a), ctypes.byref(b),
                                          ctypes.c_void_p(rowptr.data_ptr()), ctypes.byref(nnz),
-                                         None, 0, _stream_ptr(stream)), "kami_bsr_spgemm_symbolic")
+                                         ws, ws_bytes, _stream_ptr(stream)), "kami_bsr_spgemm_symbolic")
     return rowptr, int(nnz.value)
 
 
-def spgemm_fill(A: BsrMatrix, B: BsrMatrix, rowptrC, nnzbC, stream=None):
-    """Ascending block columns of every row of C."""
+def spgemm_fill(A: BsrMatrix, B: BsrMatrix, rowptrC, nnzbC, stream=None, workspace=None):
+    """Ascending block columns of every row of C (from the bitmaps the symbolic call left
+    in `workspace`, when given)."""
     import torch
     _check_pair(A, B)
     if rowptrC.dtype != torch.int32 or tuple(rowptrC.shape) != (A.mb + 1,) or \
@@ -372,9 +417,10 @@ def spgemm_fill(A: BsrMatrix, B: BsrMatrix, rowptrC, nnzbC, stream=None):
         raise ValueError(f"rowptrC must be a contiguous int32 tensor of shape ({A.mb + 1},)")
     colidx = torch.empty(max(nnzbC, 1), dtype=torch.int32, device=A.rowptr.device)
     a, b = A._c(), B._c()
+    ws, ws_bytes = _workspace_args(workspace, A.rowptr.device)
     check(lib().kami_bsr_spgemm_fill(ctypes.byref(a), ctypes.byref(b),
                                      ctypes.c_void_p(rowptrC.data_ptr()),
-                                     ctypes.c_void_p(colidx.data_ptr()), None, 0,
+                                     ctypes.c_void_p(colidx.data_ptr()), ws, ws_bytes,
                                      _stream_ptr(stream)), "kami_bsr_spgemm_fill")
     return colidx[:nnzbC]
 
@@ -420,8 +466,9 @@ def spgemm_numeric(A: BsrMatrix, B: BsrMatrix, rowptrC, colidxC, *, precision=No
 
 def bsr_spgemm(A: BsrMatrix, B: BsrMatrix, *, precision=None, alpha=1.0, stream=None):
     """C = alpha * A * B as a BsrMatrix (symbolic, fill and numeric phases)."""
-    rowptr, nnz = spgemm_symbolic(A, B, stream)
-    colidx = spgemm_fill(A, B, rowptr, nnz, stream)
+    ws = spgemm_workspace(A, B)
+    rowptr, nnz = spgemm_symbolic(A, B, stream, workspace=ws)
+    colidx = spgemm_fill(A, B, rowptr, nnz, stream, workspace=ws)
     vals = spgemm_numeric(A, B, rowptr, colidx, precision=precision, alpha=alpha, stream=stream)
     return BsrMatrix(A.mb, B.nb, A.bs, rowptr, colidx, vals)
 
@@ -462,8 +509,8 @@ def bsr_spmm(A: BsrMatrix, B, C=None, *, b_layout="nk", precision=None, alpha=1.
     return C
 
 
-__all__ += ["BsrMatrix", "spgemm_symbolic", "spgemm_fill", "spgemm_numeric", "bsr_spgemm",
-            "bsr_spmm"]
+__all__ += ["BsrMatrix", "spgemm_workspace", "spgemm_symbolic", "spgemm_fill", "spgemm_numeric",
+            "bsr_spgemm", "bsr_spmm"]
 
 
 def tune_save(path):

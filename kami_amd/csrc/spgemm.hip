@@ -5,14 +5,16 @@
 // (:806), a blocking copy of nnzC (:808), spgemm_symbolic_col_kernel (:88-116, ascending
 // column fill) and the numeric kernel block_spgemm_2d_half_mma (:171) that keeps the whole
 // matrix in one CTA.  Here every phase is its own device kernel sized for large matrices:
-//   count  -- one workgroup per block row; the row's output columns are a bitmap in LDS
-//             set with LDS atomics while walking B's rows named by A's columns; the
-//             count is the bitmap's population;
+//   count  -- one workgroup per block row; every (A entry, B entry) pair of the row is
+//             one thread's work (four dependent loads per row however long it is); the
+//             row's output columns are a bitmap in LDS set with LDS atomics, the count is
+//             its population, and with a workspace the bitmap is also stored;
 //   scan   -- one workgroup scans the counts in place into rowptrC (exclusive, with
-//             rowptrC[mb] = nnzb(C)); chunks of 1024 with a running carry;
-//   fill   -- the bitmap is rebuilt and its set bits are written in ascending order,
-//             each thread placing the bits of one 32-bit word after the prefix popcount
-//             of the words before it;
+//             rowptrC[mb] = nnzb(C)); chunks of 1024 with a running carry; the total also
+//             goes to a pinned host word the host waits for (no copy back);
+//   fill   -- the stored bitmap (without a workspace: a rebuilt one) has its set bits
+//             written in ascending order, each thread placing the bits of one 32-bit word
+//             after the prefix popcount of the words before it;
 //   numeric -- one wave per output block (i, j): the row i comes from a 64-way search
 //             of rowptrC; the lanes binary-search B's sorted rows k of all blocks (i, k)
 //             of A at once for column j; A fragments are loaded straight into
@@ -21,6 +23,7 @@
 // All integer phases are exact and deterministic.  B may have at most 491520 block
 // columns (a 60 KiB bitmap).
 #include <algorithm>
+#include <chrono>
 
 #include "kami_internal.hpp"
 
@@ -28,41 +31,91 @@ namespace kami {
 
 constexpr int kMaxBitmapWords = 15360;  // 491520 block columns: a 60 KiB bitmap
 
-// Mark the output columns of block row `row` in the LDS bitmap (bitmap must be zero).
+// Staging of one chunk of a row's A entries (256 per pass): the start of B's row named
+// by each entry and the exclusive prefix of those rows' lengths.
+struct MarkLds {
+    int b0[256];
+    int pre[256];
+    int wave_tot[4];
+};
+
+// Exclusive prefix over the workgroup's 256 threads; the total goes to *total.
+__device__ inline int block_exclusive_scan(int v, MarkLds& s, int* total) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) s.wave_tot[wave] = incl;
+    __syncthreads();
+    int before = incl - v;
+    for (int q = 0; q < wave; ++q) before += s.wave_tot[q];
+    *total = s.wave_tot[0] + s.wave_tot[1] + s.wave_tot[2] + s.wave_tot[3];
+    return before;
+}
+
+// Mark the output columns of block row `row` in the LDS bitmap (which must be zero).
+// Every (A entry, B entry) pair of the row is one unit of work spread over the
+// workgroup, so the dependent loads are four in all (A's row range; the entry's column
+// and B's row range; the pair's B column; the mark), however many entries the row has:
+// thread e takes A entry a0 + e of a chunk of 256, the chunk's B row lengths are scanned,
+// and thread p finds the entry of pair p by binary search over the prefix.
 __device__ inline void mark_row(int row, const int32_t* rpA, const int32_t* ciA,
-                                const int32_t* rpB, const int32_t* ciB, unsigned* bitmap) {
+                                const int32_t* rpB, const int32_t* ciB, unsigned* bitmap,
+                                MarkLds& s) {
+    const int tid = threadIdx.x;
     const int a0 = rpA[row], a1 = rpA[row + 1];
-    for (int t = a0; t < a1; ++t) {
-        const int k = ciA[t];
-        const int b0 = rpB[k], b1 = rpB[k + 1];
-        for (int l = b0 + (int)threadIdx.x; l < b1; l += blockDim.x) {
-            const int j = ciB[l];
+    for (int base = a0; base < a1; base += 256) {
+        const int n = min(256, a1 - base);
+        int b0 = 0, len = 0;
+        if (tid < n) {
+            const int k = ciA[base + tid];
+            b0 = rpB[k];
+            len = rpB[k + 1] - b0;
+        }
+        int pairs;
+        const int pre = block_exclusive_scan(len, s, &pairs);
+        s.b0[tid] = b0;
+        s.pre[tid] = pre;
+        __syncthreads();
+        for (int p = tid; p < pairs; p += 256) {
+            int lo = 0, hi = n - 1;  // the last entry whose prefix is <= p
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s.pre[mid] <= p) lo = mid; else hi = mid - 1;
+            }
+            const int j = ciB[s.b0[lo] + p - s.pre[lo]];
             atomicOr(&bitmap[j >> 5], 1u << (j & 31));
         }
+        __syncthreads();  // s is rewritten by the next chunk
     }
 }
 
+// Per-row output block counts of C = A * B.  With `bitmaps` (mb rows of nwords words,
+// the caller's workspace) the row's column bitmap is also stored, for the fill.
 __global__ void __launch_bounds__(256)
 spgemm_count_kernel(int mb, int nwords, const int32_t* rpA, const int32_t* ciA,
-                    const int32_t* rpB, const int32_t* ciB, int32_t* counts) {
+                    const int32_t* rpB, const int32_t* ciB, int32_t* counts, unsigned* bitmaps) {
     extern __shared__ unsigned bitmap[];
-    __shared__ int s_part[8];
+    __shared__ MarkLds s;
+    __shared__ int s_part[4];
     for (int row = blockIdx.x; row < mb; row += gridDim.x) {
         for (int w = threadIdx.x; w < nwords; w += blockDim.x) bitmap[w] = 0u;
         __syncthreads();
-        mark_row(row, rpA, ciA, rpB, ciB, bitmap);
-        __syncthreads();
+        mark_row(row, rpA, ciA, rpB, ciB, bitmap, s);
         int c = 0;
-        for (int w = threadIdx.x; w < nwords; w += blockDim.x) c += __popc(bitmap[w]);
+        for (int w = threadIdx.x; w < nwords; w += blockDim.x) {
+            const unsigned bits = bitmap[w];
+            c += __popc(bits);
+            if (bitmaps) bitmaps[(int64_t)row * nwords + w] = bits;
+        }
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off);
         if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = c;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            int tot = 0;
-            for (int q = 0; q < (int)(blockDim.x >> 6); ++q) tot += s_part[q];
-            counts[row] = tot;
-        }
+        if (threadIdx.x == 0) counts[row] = s_part[0] + s_part[1] + s_part[2] + s_part[3];
         __syncthreads();
     }
 }
@@ -73,7 +126,10 @@ spgemm_count_kernel(int mb, int nwords, const int32_t* rpA, const int32_t* ciA,
 // kept in 64 bits and every prefix is stored saturated at INT32_MAX, so an output with
 // 2^31 - 1 or more blocks (which int32 row pointers cannot hold) shows as x[n-1] ==
 // INT32_MAX instead of wrapping.
-__global__ void __launch_bounds__(1024) spgemm_scan_kernel(int32_t* x, int n) {
+// `done`, when given, is a pair of host-mapped words: done[0] receives x[n-1] and then,
+// once that store is visible to the host, done[1] receives `seq`; the host waits for seq
+// instead of copying the total back.
+__global__ void __launch_bounds__(1024) spgemm_scan_kernel(int32_t* x, int n, int32_t* done, int32_t seq) {
     __shared__ int s_wave[16];
     __shared__ long long s_carry;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -107,18 +163,30 @@ __global__ void __launch_bounds__(1024) spgemm_scan_kernel(int32_t* x, int n) {
         if (tid == 0) s_carry = carry + s_wave[15];
         __syncthreads();
     }
+    // the running total is the sum of all counts, x[n - 1] before saturation
+    if (done && tid == 0) {
+        __hip_atomic_store(&done[0], (int32_t)min(s_carry, (long long)INT32_MAX), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&done[1], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 __global__ void __launch_bounds__(256)
 spgemm_fill_kernel(int mb, int nwords, const int32_t* rpA, const int32_t* ciA,
-                   const int32_t* rpB, const int32_t* ciB, const int32_t* rpC, int32_t* ciC) {
+                   const int32_t* rpB, const int32_t* ciB, const int32_t* rpC, int32_t* ciC,
+                   const unsigned* bitmaps) {
     extern __shared__ unsigned bitmap[];
+    __shared__ MarkLds s;
     __shared__ int s_wave[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int row = blockIdx.x; row < mb; row += gridDim.x) {
-        for (int w = tid; w < nwords; w += blockDim.x) bitmap[w] = 0u;
-        __syncthreads();
-        mark_row(row, rpA, ciA, rpB, ciB, bitmap);
+        if (bitmaps) {  // the count kernel's bitmap of the row
+            for (int w = tid; w < nwords; w += blockDim.x) bitmap[w] = bitmaps[(int64_t)row * nwords + w];
+        } else {
+            for (int w = tid; w < nwords; w += blockDim.x) bitmap[w] = 0u;
+            __syncthreads();
+            mark_row(row, rpA, ciA, rpB, ciB, bitmap, s);
+        }
         __syncthreads();
         int out = rpC[row];
         for (int base = 0; base < nwords; base += 256) {
@@ -297,6 +365,54 @@ static kami_status check_pair(const kami_bsr* A, const kami_bsr* B) {
     return KAMI_OK;
 }
 
+// Bytes of the per-row column bitmaps of C = A * B (A with mb block rows, B with nb
+// block columns).
+static size_t bitmap_bytes(int32_t mb, int32_t nb) {
+    return (size_t)mb * (size_t)((nb + 31) / 32) * sizeof(unsigned);
+}
+
+// Two pinned, device-visible host words per host thread for the symbolic phase's total
+// and its sequence number (the scan kernel writes both; no copy back).  Kept until the
+// process ends; null when pinned memory cannot be had.
+static int32_t* pinned_total() {
+    thread_local int32_t* word = nullptr;
+    thread_local bool tried = false;
+    if (!tried) {
+        tried = true;
+        void* p = nullptr;
+        if (hipHostMalloc(&p, 64, hipHostMallocPortable | hipHostMallocMapped) == hipSuccess)
+            word = static_cast<int32_t*>(p);
+        else
+            (void)hipGetLastError();
+    }
+    return word;
+}
+
+static int32_t* pinned_device(int32_t* host) {
+    void* d = nullptr;
+    if (hipHostGetDevicePointer(&d, host, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return host;  // unified addressing: the host address is the device address
+    }
+    return static_cast<int32_t*>(d);
+}
+
+// Waits until the device has stored `seq` in the pinned word: a short spin on the word
+// (the kernels before it take microseconds), then the stream's own blocking wait, so that
+// a long queue of earlier work on the stream does not keep a host core spinning.
+static hipError_t wait_for_word(const int32_t* word, int32_t seq, hipStream_t s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    while (__atomic_load_n(word, __ATOMIC_ACQUIRE) != seq) {
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(200)) {
+            const hipError_t e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return e;
+            // the stream is done, so the scan kernel has stored the word
+            return __atomic_load_n(word, __ATOMIC_ACQUIRE) == seq ? hipSuccess : hipErrorUnknown;
+        }
+    }
+    return hipSuccess;
+}
+
 static int sym_grid(int mb) {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess)
@@ -312,14 +428,12 @@ extern "C" {
 
 kami_status kami_bsr_spgemm_workspace_size(int32_t mb, int32_t nb, size_t* bytes) {
     if (mb <= 0 || nb <= 0 || !bytes) return KAMI_EINVAL;
-    *bytes = 0;  // counts are scanned in place in rowptrC
+    *bytes = bitmap_bytes(mb, nb);  // one column bitmap per block row of C
     return KAMI_OK;
 }
 
 kami_status kami_bsr_spgemm_symbolic(const kami_bsr* A, const kami_bsr* B, int32_t* rowptrC,
                                      int64_t* nnzbC, void* ws, size_t ws_bytes, void* stream) {
-    (void)ws;
-    (void)ws_bytes;
     kami_status st = check_pair(A, B);
     if (st != KAMI_OK) return st;
     if (!rowptrC || !nnzbC) return KAMI_EINVAL;
@@ -327,15 +441,26 @@ kami_status kami_bsr_spgemm_symbolic(const kami_bsr* A, const kami_bsr* B, int32
     if (st != KAMI_OK) return st;
     hipStream_t s = (hipStream_t)stream;
     const int nwords = (B->nb + 31) / 32;
+    unsigned* bitmaps = ws && ws_bytes >= bitmap_bytes(A->mb, B->nb) ? static_cast<unsigned*>(ws) : nullptr;
+    int32_t* words = pinned_total();
+    thread_local int32_t seq = 0;
+    seq = seq == INT32_MAX ? 1 : seq + 1;
     hipLaunchKernelGGL(spgemm_count_kernel, dim3(sym_grid(A->mb)), dim3(256), nwords * 4, s, A->mb,
-                       nwords, A->rowptr, A->colidx, B->rowptr, B->colidx, rowptrC);
-    hipLaunchKernelGGL(spgemm_scan_kernel, dim3(1), dim3(1024), 0, s, rowptrC, A->mb + 1);
+                       nwords, A->rowptr, A->colidx, B->rowptr, B->colidx, rowptrC, bitmaps);
+    hipLaunchKernelGGL(spgemm_scan_kernel, dim3(1), dim3(1024), 0, s, rowptrC, A->mb + 1,
+                       words ? pinned_device(words) : nullptr, seq);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e);
     int32_t total = 0;
-    e = hipMemcpyAsync(&total, rowptrC + A->mb, sizeof(int32_t), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return hip_fail(e);
+    if (words) {
+        e = wait_for_word(words + 1, seq, s);
+        if (e != hipSuccess) return hip_fail(e);
+        total = __atomic_load_n(words, __ATOMIC_ACQUIRE);
+    } else {  // no pinned words: copy the total back
+        e = hipMemcpyAsync(&total, rowptrC + A->mb, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(e);
+    }
     if (total == INT32_MAX) return KAMI_ENOTSUP;  // more blocks than int32 row pointers hold
     *nnzbC = total;
     return KAMI_OK;
@@ -343,17 +468,17 @@ kami_status kami_bsr_spgemm_symbolic(const kami_bsr* A, const kami_bsr* B, int32
 
 kami_status kami_bsr_spgemm_fill(const kami_bsr* A, const kami_bsr* B, const int32_t* rowptrC,
                                  int32_t* colidxC, void* ws, size_t ws_bytes, void* stream) {
-    (void)ws;
-    (void)ws_bytes;
     kami_status st = check_pair(A, B);
     if (st != KAMI_OK) return st;
     if (!rowptrC || !colidxC) return KAMI_EINVAL;
     st = device_check();
     if (st != KAMI_OK) return st;
     const int nwords = (B->nb + 31) / 32;
+    const unsigned* bitmaps =
+        ws && ws_bytes >= bitmap_bytes(A->mb, B->nb) ? static_cast<const unsigned*>(ws) : nullptr;
     hipLaunchKernelGGL(spgemm_fill_kernel, dim3(sym_grid(A->mb)), dim3(256), nwords * 4,
                        (hipStream_t)stream, A->mb, nwords, A->rowptr, A->colidx, B->rowptr,
-                       B->colidx, rowptrC, colidxC);
+                       B->colidx, rowptrC, colidxC, bitmaps);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e);
     return KAMI_OK;

@@ -42,14 +42,17 @@ def random_pattern(rng, mb, nb, p):
 
 
 def check_structure(A, B, rpA, ciA, rpB, ciB):
-    rowptr, nnz = K.spgemm_symbolic(A, B)
-    colidx = K.spgemm_fill(A, B, rowptr, nnz)
-    torch.cuda.synchronize()
+    """Structure bit-exact against the oracle, without a workspace (the fill rebuilds
+    the bitmaps) and with one (the fill expands the bitmaps the symbolic call stored)."""
     rp_ref, nnz_ref = oracle.spgemm_symbolic(A.mb, B.nb, rpA, ciA, rpB, ciB)
-    np.testing.assert_array_equal(rowptr.cpu().numpy(), rp_ref)
-    assert nnz == nnz_ref
     ci_ref = oracle.spgemm_fill(A.mb, B.nb, rpA, ciA, rpB, ciB, rp_ref)
-    np.testing.assert_array_equal(colidx.cpu().numpy(), ci_ref)
+    for ws in (None, K.spgemm_workspace(A, B)):
+        rowptr, nnz = K.spgemm_symbolic(A, B, workspace=ws)
+        colidx = K.spgemm_fill(A, B, rowptr, nnz, workspace=ws)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(rowptr.cpu().numpy(), rp_ref)
+        assert nnz == nnz_ref
+        np.testing.assert_array_equal(colidx.cpu().numpy(), ci_ref)
     return rowptr, colidx, rp_ref, nnz
 
 
@@ -167,3 +170,39 @@ def test_many_rows_multi_chunk_scan(dev):
     A = to_bsr(mb, 50, 16, rpA, ciA, np.ones((len(ciA), 16, 16)), dev)
     B = to_bsr(50, 60, 16, rpB, ciB, np.ones((len(ciB), 16, 16)), dev)
     check_structure(A, B, rpA, ciA, rpB, ciB)
+
+
+def test_long_rows_cross_the_pair_chunks(dev):
+    """Rows of A with more than 256 blocks and rows whose (A block, B block) pairs number
+    several thousand: the symbolic kernels take A's entries 256 at a time and spread the
+    pairs over the workgroup; rows of every length from empty to dense."""
+    rng = np.random.default_rng(6)
+    mb, kb, nb = 24, 700, 300
+    dens = np.linspace(0.0, 1.0, mb)[:, None]          # row i of A at density i / (mb - 1)
+    maskA = rng.random((mb, kb)) < dens
+    rpA = np.concatenate([[0], np.cumsum(maskA.sum(1))]).astype(np.int32)
+    ciA = np.nonzero(maskA)[1].astype(np.int32)
+    rpB, ciB = random_pattern(rng, kb, nb, 0.05)
+    A = to_bsr(mb, kb, 16, rpA, ciA, np.ones((len(ciA), 16, 16)), dev)
+    B = to_bsr(kb, nb, 16, rpB, ciB, np.ones((len(ciB), 16, 16)), dev)
+    assert (rpA[1:] - rpA[:-1]).max() > 600
+    check_structure(A, B, rpA, ciA, rpB, ciB)
+
+
+def test_workspace_smaller_than_the_bitmaps_is_not_used(dev):
+    """A workspace below the bitmap size is ignored: the fill rebuilds the bitmaps, and
+    a stale larger workspace of another product is never read when too small."""
+    rng = np.random.default_rng(7)
+    rpA, ciA = random_pattern(rng, 50, 40, 0.2)
+    rpB, ciB = random_pattern(rng, 40, 70, 0.2)
+    A = to_bsr(50, 40, 16, rpA, ciA, np.ones((len(ciA), 16, 16)), dev)
+    B = to_bsr(40, 70, 16, rpB, ciB, np.ones((len(ciB), 16, 16)), dev)
+    full = K.spgemm_workspace(A, B)
+    small = torch.full((full.numel() - 4,), 0xFF, dtype=torch.uint8, device=dev)
+    rowptr, nnz = K.spgemm_symbolic(A, B, workspace=small)
+    colidx = K.spgemm_fill(A, B, rowptr, nnz, workspace=small)
+    rp_ref, nnz_ref = oracle.spgemm_symbolic(50, 70, rpA, ciA, rpB, ciB)
+    assert nnz == nnz_ref
+    np.testing.assert_array_equal(colidx.cpu().numpy(),
+                                  oracle.spgemm_fill(50, 70, rpA, ciA, rpB, ciB, rp_ref))
+    assert torch.all(small == 0xFF)
