@@ -88,6 +88,17 @@ void register_f16_instances(std::vector<BlockInstance>& v) {
     add_instance<A, F, 192, 192, 32, 12, 1, 1>(v, 1, 12);
     add_instance<A, F, 192, 192, 32, 4, 4, 1>(v, 2, 4);
 
+    // 208^3 and 224^3 (the reference's FP16 square sweep reaches 224,
+    // logs/block_gemm/square/H200/fp16_block_square_1d_H200.csv:13-14): 1D, 13 and 14
+    // waves of 16 rows; B's whole K (90-107 KB) exceeds the stage budget, so K is staged
+    // 16 (208) or 32 (224) wide through double-buffered LDS
+    add_instance<H, F, 208, 208, 208, 13, 1, 1, 2>(v, 1, 13);  // K step 16: 208 = 6.5 x 32
+    add_instance<A, F, 224, 224, 224, 14, 1, 1, 2>(v, 1, 14);
+    // K = 512 folded into LDS stages (src/batched_gemm/KAMI/1d/batched_gemm_1d_double_mma_128.cu:100-205)
+    add_instance<A, F, 128, 128, 512, 8, 1, 1>(v, 1, 8);
+    add_instance<A, F, 128, 128, 512, 4, 4, 1>(v, 2, 4);
+    add_instance<A, F, 256, 256, 512, 16, 1, 1>(v, 1, 16);
+
     // 3D at 192^3, 256^3 and 256^2 x {16, 32} (2 x 2 x 2 waves) in column passes: the
     // full-width wave tiles (96 x 96 and 128 x 128 FP32 per wave at two depths) exceed
     // the register file, tiles of BN / SP columns do not

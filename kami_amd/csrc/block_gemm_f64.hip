@@ -33,6 +33,11 @@ void register_f64_instances(std::vector<BlockInstance>& v) {
     add_instance<A, F, 128, 128, 128, 2, 2, 1>(v, 2, 2);
     add_instance<A, F, 128, 128, 128, 4, 4, 1>(v, 2, 4);
     add_instance<A, F, 128, 128, 128, 2, 2, 2>(v, 3, 2);
+    // K beyond one register/LDS stage: K = 512 folded into double-buffered LDS stages
+    // (the reference's K fold, src/batched_gemm/KAMI/1d/batched_gemm_1d_double_mma_128.cu:100-205)
+    add_instance<A, F, 64, 64, 512, 2, 2, 1>(v, 2, 2);
+    add_instance<A, F, 128, 128, 512, 8, 1, 1>(v, 1, 8);
+    add_instance<A, F, 128, 128, 512, 4, 4, 1>(v, 2, 4);
     // persistent 2D forms (pipeline 2)
     add_pipe_instance<A, F, 32, 32, 32, 2, 2, 4>(v, 2);
     add_pipe_instance<A, F, 64, 64, 64, 2, 2, 4>(v, 2);

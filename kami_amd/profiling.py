@@ -6,6 +6,7 @@
     python -m kami_amd.profiling w3 --path split         # same through FP16 slices
     python -m kami_amd.profiling w3 --path split_i8      # same through int8 digits
     python -m kami_amd.profiling w1 | w3r | w4 --prec f16 | w5 | w6
+    python -m kami_amd.profiling w2 --shape 64x64x64 --part 3d --iters 200  # FP16 resident repeat
 
 Each run launches the kernel `--reps` times after one warm-up, so per-dispatch counters
 (FETCH_SIZE, WRITE_SIZE in KiB) can be divided by the algorithmic bytes of one launch.
@@ -22,7 +23,11 @@ import numpy as np
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["copy", "add", "w1", "w3", "w3r", "w4", "w5", "w6"])
+    ap.add_argument("what", choices=["copy", "add", "w1", "w2", "w3", "w3r", "w4", "w5", "w6"])
+    ap.add_argument("--shape", default="64x64x64")
+    ap.add_argument("--part", choices=["1d", "2d", "3d"], default="2d")
+    ap.add_argument("--iters", type=int, default=200)
+    ap.add_argument("--count", type=int, default=2048)
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--path", choices=["native", "split", "split_i8"], default="native")
     ap.add_argument("--prec", choices=["f16", "f64"], default="f16")
@@ -54,6 +59,21 @@ def main(argv=None):
             torch.add(a, b, out=c)
         torch.cuda.synchronize()
         print(f"add: {args.reps + 1} launches of 2 x {args.mib} MiB read + {args.mib} MiB written")
+        return
+    if args.what == "w2":
+        # FP16 block GEMM in the resident-repeat mode (the reference's NUM_ITER loop):
+        # N(0,1) inputs, `iters` products per launch on `count` problems
+        m, n, k = (int(x) for x in args.shape.split("x"))
+        g = torch.Generator(device=dev)
+        g.manual_seed(W.SEED + 1)
+        A = torch.randn((args.count, m, k), generator=g, device=dev).half()
+        B = torch.randn((args.count, n, k), generator=g, device=dev).half()
+        C = torch.empty((args.count, m, n), dtype=torch.float32, device=dev)
+        for _ in range(args.reps + 1):
+            K.block_gemm(A, B, C, partition=args.part, iters=args.iters)
+        torch.cuda.synchronize()
+        flop = 2.0 * m * n * k * args.count * args.iters
+        print(f"w2 {args.shape} {args.part}: {args.reps + 1} launches, {flop:.4g} FLOP each")
         return
     if args.what == "w3":
         rng = np.random.default_rng(W.SEED + 2)

@@ -19,16 +19,20 @@ pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
 
-SQUARE = {K.PREC_F64: [16, 32, 64, 128], K.PREC_F16_F32: [16, 32, 64, 128, 192, 256],
-          K.PREC_BF16_F32: [16, 32, 64, 128, 192, 256]}
+SQUARE = {K.PREC_F64: [16, 32, 64, 128], K.PREC_F16_F32: [16, 32, 64, 128, 192, 208, 224, 256],
+          K.PREC_BF16_F32: [16, 32, 64, 128, 192, 208, 224, 256]}
 LOWRANK = [(s, s, k) for s in (32, 64, 128, 192, 256) for k in (16, 32) if s != k]
+# K beyond one stage, folded through LDS
+KFOLD = {K.PREC_F64: [(64, 64, 512), (128, 128, 512)],
+         K.PREC_F16_F32: [(128, 128, 512), (256, 256, 512)],
+         K.PREC_BF16_F32: [(128, 128, 512), (256, 256, 512)]}
 
 
 def instances(prec, shapes):
     out = []
     for (m, n, k) in shapes:
         for part in ("1d", "2d", "3d"):
-            for r in (1, 2, 4, 8, 12, 16):
+            for r in (1, 2, 4, 8, 12, 13, 14, 16):
                 try:
                     K.query(m, n, k, prec, partition=part, ranks=r)
                 except K.KamiError:
@@ -37,9 +41,11 @@ def instances(prec, shapes):
     return out
 
 
-F64_CASES = instances(K.PREC_F64, [(s, s, s) for s in SQUARE[K.PREC_F64]])
-F16_CASES = instances(K.PREC_F16_F32, [(s, s, s) for s in SQUARE[K.PREC_F16_F32]] + LOWRANK)
-BF16_CASES = instances(K.PREC_BF16_F32, [(s, s, s) for s in SQUARE[K.PREC_BF16_F32]] + LOWRANK)
+F64_CASES = instances(K.PREC_F64, [(s, s, s) for s in SQUARE[K.PREC_F64]] + KFOLD[K.PREC_F64])
+F16_CASES = instances(K.PREC_F16_F32, [(s, s, s) for s in SQUARE[K.PREC_F16_F32]] + LOWRANK +
+                      KFOLD[K.PREC_F16_F32])
+BF16_CASES = instances(K.PREC_BF16_F32, [(s, s, s) for s in SQUARE[K.PREC_BF16_F32]] + LOWRANK +
+                       KFOLD[K.PREC_BF16_F32])
 
 
 def ids(cases):
