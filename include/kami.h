@@ -86,7 +86,9 @@ typedef struct kami_gemm_desc {
                           epilogue, as the reference's NUM_ITER repeat does; 1 = plain GEMM.
                           Split precision takes iters > 1 at 32^3 and 64^3 (FP16/BF16) and
                           64^3 and 128^3 (int8): digits made once, the slice products repeated */
-    int64_t lda, ldb, ldc; /* elements; 0 = packed */
+    int64_t lda, ldb, ldc; /* elements; 0 = packed.  Every entry point refuses (KAMI_EINVAL)
+                              an operand whose rows x leading dimension, or a batch whose
+                              count x stride, spans more than 2^48 bytes */
     double alpha, beta;    /* C = alpha * (A B) + beta * C */
     int32_t slices;        /* split precision: slices per operand, 0 = default */
     int32_t pipeline;      /* 0 = tuned choice; 1 = one workgroup per problem;

@@ -723,6 +723,9 @@ static kami_status check_problem(kami_problem& p, int b_layout, int ei, int eo) 
     if (!p.ldc) p.ldc = p.n;
     if (p.lda < p.k || p.ldb < (b_layout == KAMI_B_NK ? p.k : p.n) || p.ldc < p.n)
         return KAMI_EINVAL;
+    if (!span_fits(p.m, p.lda, ei) || !span_fits(b_layout == KAMI_B_NK ? p.n : p.k, p.ldb, ei) ||
+        !span_fits(p.m, p.ldc, eo))
+        return KAMI_EINVAL;
     if ((p.lda * ei) % 16 || (p.ldb * ei) % 16 || ((uintptr_t)p.A % 16) || ((uintptr_t)p.B % 16) ||
         ((uintptr_t)p.C % eo))
         return KAMI_EINVAL;

@@ -162,6 +162,9 @@ kami_status validate_desc(const kami_gemm_desc* d) {
     if (ldb < (d->b_layout == KAMI_B_NK ? d->k : d->n)) return KAMI_EINVAL;
     if (ldc < d->n) return KAMI_EINVAL;
     const int e = elt_in(d->precision);
+    if (!span_fits(d->m, lda, e) || !span_fits(d->b_layout == KAMI_B_NK ? d->n : d->k, ldb, e) ||
+        !span_fits(d->m, ldc, elt_out(d->precision)))
+        return KAMI_EINVAL;
     // operands are staged with 16-byte loads
     if ((lda * e) % 16 != 0 || (ldb * e) % 16 != 0) return KAMI_EINVAL;
     return KAMI_OK;
@@ -289,7 +292,12 @@ using namespace kami;
 extern "C" {
 
 const char* kami_status_string(kami_status s) {
-    switch (s) {
+    // A C caller may pass any int; read the bits as one rather than as the C++ enum,
+    // whose value range ends at 7.
+    static_assert(sizeof(kami_status) == sizeof(int32_t), "kami_status is an int");
+    int32_t v;
+    std::memcpy(&v, &s, sizeof v);
+    switch (v) {
         case KAMI_OK: return "KAMI_OK: success";
         case KAMI_EINVAL: return "KAMI_EINVAL: invalid argument";
         case KAMI_ENOTSUP: return "KAMI_ENOTSUP: no compiled instance for this request";
@@ -372,6 +380,10 @@ static kami_status prepare_args(const kami_gemm_desc* d, const void* A, const vo
     args.sa = stride_a ? stride_a : (d->lda ? a_span : (int64_t)d->m * d->k);
     args.sb = stride_b ? stride_b : (d->ldb ? b_span : (int64_t)d->n * d->k);
     args.sc = stride_c ? stride_c : (d->ldc ? c_span : (int64_t)d->m * d->n);
+    // the whole batch within the span limit (each operand already is)
+    if (!span_fits(count, args.sa, ei) || !span_fits(count, args.sb, ei) ||
+        !span_fits(count, args.sc, eo))
+        return KAMI_EINVAL;
     if (count > 1 && ((args.sa * ei) % 16 || (args.sb * ei) % 16)) return KAMI_EINVAL;
     // problems whose C regions overlap would race on the shared elements
     if (count > 1 && args.sc < c_span) return KAMI_EINVAL;

@@ -33,6 +33,15 @@ void clear_batched_tuned_variants();
 int elt_in(int prec);
 int elt_out(int prec);
 
+// Largest byte span one operand (or one batch of operands) may describe: 2^48 bytes,
+// far beyond any device's memory.  Validators reject larger shapes before multiplying,
+// so every offset computed from a validated shape stays inside int64.
+constexpr int64_t kMaxSpanBytes = (int64_t)1 << 48;
+// rows x ld elements of elt bytes within kMaxSpanBytes; rows >= 1, ld >= 0, elt >= 1
+inline bool span_fits(int64_t rows, int64_t ld, int64_t elt) {
+    return rows >= 1 && ld >= 0 && ld <= kMaxSpanBytes / elt / rows;
+}
+
 inline bool is_split(int prec) {
     return prec == KAMI_PREC_F64_SPLIT_F16 || prec == KAMI_PREC_F64_SPLIT_BF16 ||
            prec == KAMI_PREC_F64_SPLIT_I8;

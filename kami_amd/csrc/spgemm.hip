@@ -368,7 +368,7 @@ static kami_status check_pair(const kami_bsr* A, const kami_bsr* B) {
 // Bytes of the per-row column bitmaps of C = A * B (A with mb block rows, B with nb
 // block columns).
 static size_t bitmap_bytes(int32_t mb, int32_t nb) {
-    return (size_t)mb * (size_t)((nb + 31) / 32) * sizeof(unsigned);
+    return (size_t)mb * (((size_t)nb + 31) / 32) * sizeof(unsigned);
 }
 
 // Two pinned, device-visible host words per host thread for the symbolic phase's total

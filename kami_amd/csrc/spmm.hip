@@ -392,10 +392,13 @@ kami_status kami_bsr_spmm(const kami_bsr* A, const void* B, int32_t n, int64_t l
     if (ldc == 0) ldc = n;
     if ((b_layout == KAMI_B_NK && ldb < K) || (b_layout == KAMI_B_KN && ldb < n) || ldc < n)
         return KAMI_EINVAL;
+    if (!span_fits(b_layout == KAMI_B_NK ? n : K, ldb, ei) ||
+        !span_fits((int64_t)A->mb * A->bs, ldc, eo))
+        return KAMI_EINVAL;
     if ((ldb * ei) % 16 || ((uintptr_t)B % 16) || ((uintptr_t)C % eo)) return KAMI_EINVAL;
     if (A->nnzb > 0 && ((uintptr_t)A->vals % 16)) return KAMI_EINVAL;
     // workgroups: one per (block row, 64-column tile of C) in both forms
-    const int64_t wgs = (int64_t)A->mb * ((n + 63) / 64);
+    const int64_t wgs = (int64_t)A->mb * (((int64_t)n + 63) / 64);
     if (wgs >= ((int64_t)1 << 24)) return KAMI_ENOTSUP;  // work-items of one dispatch < 2^32
     kami_status st = device_check();
     if (st != KAMI_OK) return st;

@@ -12,6 +12,7 @@
 //   block <precision> <m> <n> <k> <partition> <ranks> <pipeline>
 //   batched <precision> <bucket> <variant>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -86,20 +87,34 @@ kami_status kami_tune_load(const char* path) {
     if (!f) return KAMI_EINVAL;
     std::vector<TunedBlock> blocks;
     std::vector<int> batched;  // prec, bucket, variant triples
-    char line[256];
+    char* line = nullptr;
+    size_t cap = 0;
+    ssize_t len;
     bool header = false, ok = true;
-    while (ok && std::fgets(line, sizeof line, f)) {
+    // a record is the whole line: its fields, then only white space
+    auto rest_blank = [](const char* p) {
+        while (*p == ' ' || *p == '\t' || *p == '\r' || *p == '\n') ++p;
+        return *p == '\0';
+    };
+    while (ok && (len = getline(&line, &cap, f)) != -1) {
+        if ((size_t)len != std::strlen(line)) {  // a NUL byte inside the line
+            ok = false;
+            break;
+        }
         if (line[0] == '\n' || line[0] == '#') continue;
+        int used = -1;
         if (!header) {
             int ver = 0;
-            ok = std::sscanf(line, "kami-tune %d", &ver) == 1 && ver == 1;
+            ok = std::sscanf(line, "kami-tune %d%n", &ver, &used) == 1 && used > 0 &&
+                 rest_blank(line + used) && ver == 1;
             header = true;
             continue;
         }
         TunedBlock t;
         int prec, bucket, variant;
-        if (std::sscanf(line, "block %d %d %d %d %d %d %d", &t.prec, &t.m, &t.n, &t.k, &t.part,
-                        &t.ranks, &t.pipe) == 7) {
+        if (std::sscanf(line, "block %d %d %d %d %d %d %d%n", &t.prec, &t.m, &t.n, &t.k, &t.part,
+                        &t.ranks, &t.pipe, &used) == 7 &&
+            used > 0 && rest_blank(line + used)) {
             // the choice must name a compiled instance (checked like a query)
             kami_gemm_desc d;
             ok = kami_gemm_desc_init(&d, t.m, t.n, t.k, t.prec) == KAMI_OK && !is_split(t.prec) &&
@@ -113,7 +128,8 @@ kami_status kami_tune_load(const char* path) {
                 ok = resolve_block(&d, &st) != nullptr;
             }
             if (ok) blocks.push_back(t);
-        } else if (std::sscanf(line, "batched %d %d %d", &prec, &bucket, &variant) == 3) {
+        } else if (std::sscanf(line, "batched %d %d %d%n", &prec, &bucket, &variant, &used) == 3 &&
+                   used > 0 && rest_blank(line + used)) {
             ok = batched_prec_ok(prec) && bucket >= 0 && bucket < kBatchedBuckets &&
                  variant >= 0 && variant < kBatchedVariants;
             if (ok) batched.insert(batched.end(), {prec, bucket, variant});
@@ -121,6 +137,7 @@ kami_status kami_tune_load(const char* path) {
             ok = false;
         }
     }
+    std::free(line);
     std::fclose(f);
     if (!ok || !header) return KAMI_EINVAL;  // nothing is applied from a bad file
     for (const TunedBlock& t : blocks) set_tuned_choice(t.prec, t.m, t.n, t.k, t.part, t.ranks, t.pipe);

@@ -124,6 +124,7 @@ kami_status kami_split_slices(const double* X, int64_t rows, int64_t k, int64_t 
     if (ldx == 0) ldx = k;
     if (ldx < k) return KAMI_EINVAL;
     if (rows == 0 || k == 0) return KAMI_OK;
+    if (!span_fits(rows, ldx, 8) || !span_fits(rows, k, 2 * (int64_t)slices)) return KAMI_EINVAL;
     // columns span one grid dimension of 256-thread workgroups, below 2^32 work-items
     if ((k + 255) / 256 * 256 >= ((int64_t)1 << 32)) return KAMI_ENOTSUP;
     kami_status st = device_check();
@@ -149,6 +150,8 @@ kami_status kami_generate(int32_t kind, int32_t precision, void* dst, int64_t co
     const size_t align = precision == KAMI_PREC_F64 ? 8 : 2;
     if ((uintptr_t)dst % align) return KAMI_EINVAL;
     if (count == 0) return KAMI_OK;
+    // the values' indices offset + i stay inside int64
+    if (!span_fits(count, 1, (int64_t)align) || offset > INT64_MAX - count) return KAMI_EINVAL;
     kami_status st = device_check();
     if (st != KAMI_OK) return st;
     GenArgs a{dst, count, offset, kind, precision};
