@@ -56,11 +56,15 @@ template <class AT>
 struct StreamGeom {
     static constexpr int NT = 256;
     static constexpr int ELT = (int)sizeof(typename AT::in_t), VE = 16 / ELT;
-    // FP64 stages 16-wide K chunks of tiles up to 64 x 64 (the 128-row variant is not
-    // offered), FP16/BF16 64-wide chunks of tiles up to 128 x 64: 40 KB / 60 KB of LDS for
-    // the two stage buffers (160-byte rows), three / two workgroups per CU
-    static constexpr int KC = ELT == 8 ? 16 : 64;
-    static constexpr int TMAX = ELT == 8 ? 64 : 128, TNMAX = 64;
+    // FP64 stages 8-wide K chunks of tiles up to 128 x 128 (96-byte rows, 48 KB of LDS
+    // for the two stage buffers), FP16/BF16 64-wide chunks of tiles up to 128 x 64
+    // (160-byte rows, 60 KB); two workgroups per CU either way.  FP64 offers the 128-wide
+    // tiles because its batches are bound by HBM: a problem cut into 64 x 64 tiles reads
+    // each row of A once per 64 columns of C and each row of B once per 64 rows, and
+    // across the thousands of tiles in flight L2 keeps none of those rows for the next
+    // tile that needs them
+    static constexpr int KC = ELT == 8 ? 8 : 64;
+    static constexpr int TMAX = 128, TNMAX = ELT == 8 ? 128 : 64;
     static constexpr int ROWB = lds_row_bytes(KC * ELT, AT::FRAG_BYTES);
     static constexpr int A_REGION = TMAX * ROWB;
     static constexpr int STAGE = (TMAX + TNMAX) * ROWB;
@@ -85,9 +89,11 @@ struct TileGeom {
 // Tile variants.  Index = variant id stored in the work items.
 //   0: 64 x 64 tile, 2 x 2 waves (2D)       1: 32 x 32 tile, 2 x 2 waves (2D)
 //   2: 64 x 64 tile, 4 x 1 waves (1D)       3: 128 x 64 tile, 2 x 2 waves (2D)
-constexpr int kVariants = 4;
-static const int kVariantTM[kVariants] = {64, 32, 64, 128};
-static const int kVariantTN[kVariants] = {64, 32, 64, 64};
+//   4: 128 x 128 tile, 2 x 2 waves (FP64)   5: 64 x 128 tile, 2 x 2 waves (FP64)
+constexpr int kVariants = 6;
+static_assert(kVariants == kBatchedVariants, "tuning-cache variant range");
+static const int kVariantTM[kVariants] = {64, 32, 64, 128, 128, 64};
+static const int kVariantTN[kVariants] = {64, 32, 64, 64, 128, 128};
 
 template <class AT>
 struct Variants {
@@ -95,13 +101,15 @@ struct Variants {
     using V1 = TileGeom<AT, 32, 32, 2, 2>;
     using V2 = TileGeom<AT, 64, 64, 4, 1>;
     using V3 = TileGeom<AT, 128, 64, 2, 2>;
+    using V4 = TileGeom<AT, 128, 128, 2, 2>;
+    using V5 = TileGeom<AT, 64, 128, 2, 2>;
 };
 
-// Variants a precision offers: FP64 stages at most 64 rows, so not variant 3.
-inline int variants_of(int prec) { return prec == KAMI_PREC_F64 ? 3 : kVariants; }
+// Variants a precision offers: FP16/BF16 stage 64 columns of B, so not 4 and 5.
+inline int variants_of(int prec) { return prec == KAMI_PREC_F64 ? kVariants : 4; }
 
-__device__ inline int variant_tm(int v) { return v == 1 ? 32 : (v == 3 ? 128 : 64); }
-__device__ inline int variant_tn(int v) { return v == 1 ? 32 : 64; }
+__device__ inline int variant_tm(int v) { return v == 1 ? 32 : (v == 3 || v == 4 ? 128 : 64); }
+__device__ inline int variant_tn(int v) { return v == 1 ? 32 : (v == 4 || v == 5 ? 128 : 64); }
 
 struct BatchArgs {
     const WorkItem* items;
@@ -403,30 +411,54 @@ __device__ __forceinline__ void run_tile(const BatchArgs& a, Stream<AT>& st, con
     // and a guard per atom would make the compiler copy the accumulators around each
     // MFMA.  Waves whose whole sub-tile lies outside C skip the products.
     const bool wave_live = wm * V::SUBM < mlim && wn * V::SUBN < nlim;  // wave-uniform
+    // atoms of the wave's sub-tile inside C (wave-uniform)
+    const int im_live = uni(min(V::IM, (mlim - wm * V::SUBM + 15) / 16));
+    const int jn_live = uni(min(V::JN, (nlim - wn * V::SUBN + 15) / 16));
     auto compute = [&](const unsigned char* base) {
         const unsigned char* bb = base + SG::A_REGION;
         if (!wave_live) return;
 #pragma unroll
         for (int ks = 0; ks < SG::KC / AT::KSTEP; ++ks) {
             const int kk = (ks * AT::KSTEP + AT::KV * (lane >> 4)) * SG::ELT;
-            frag_t af[V::IM], bf[V::JN];
-#pragma unroll
-            for (int im = 0; im < V::IM; ++im)
-                af[im] = bitcast<frag_t>(load_raw<FB>(
-                    base + (wm * V::SUBM + im * 16 + (lane & 15)) * SG::ROWB + kk));
+            frag_t bf[V::JN];
 #pragma unroll
             for (int jn = 0; jn < V::JN; ++jn)
                 bf[jn] = bitcast<frag_t>(load_raw<FB>(
                     bb + (wn * V::SUBN + jn * 16 + (lane & 15)) * SG::ROWB + kk));
-            // sub-steps of a fragment (FP64: two 16x16x4 instructions) interleaved across
-            // the accumulators, so no instruction waits on the one just issued
+            auto a_frag = [&](int im) {
+                return bitcast<frag_t>(load_raw<FB>(
+                    base + (wm * V::SUBM + im * 16 + (lane & 15)) * SG::ROWB + kk));
+            };
+            if constexpr (V::IM * V::JN >= 16) {
+                // the 128 x 128 FP64 tile: its 128 accumulator VGPRs leave room for one
+                // A fragment at a time; the JN products of a fragment's sub-step separate
+                // the two sub-steps of one accumulator.  Row and column atoms past the
+                // problem's edge are skipped (wave-uniform branches): M and N are
+                // multiples of 16, so no product is wasted
 #pragma unroll
-            for (int e = 0; e < AT::NSUB; ++e)
+                for (int im = 0; im < V::IM; ++im) {
+                    if (im >= im_live) break;
+                    const frag_t af = a_frag(im);
 #pragma unroll
-                for (int jn = 0; jn < V::JN; ++jn)
+                    for (int e = 0; e < AT::NSUB; ++e)
 #pragma unroll
-                    for (int im = 0; im < V::IM; ++im)
-                        AT::mma_sub(acc[im][jn], af[im], bf[jn], e);
+                        for (int jn = 0; jn < V::JN; ++jn)
+                            if (jn < jn_live) AT::mma_sub(acc[im][jn], af, bf[jn], e);
+                }
+            } else {
+                frag_t af[V::IM];
+#pragma unroll
+                for (int im = 0; im < V::IM; ++im) af[im] = a_frag(im);
+                // sub-steps of a fragment (FP64: two 16x16x4 instructions) interleaved
+                // across the accumulators, so no instruction waits on the one just issued
+#pragma unroll
+                for (int e = 0; e < AT::NSUB; ++e)
+#pragma unroll
+                    for (int jn = 0; jn < V::JN; ++jn)
+#pragma unroll
+                        for (int im = 0; im < V::IM; ++im)
+                            AT::mma_sub(acc[im][jn], af[im], bf[jn], e);
+            }
         }
     };
     // stage of parity P: products from LDS buffer P, then the next stage (slot 1 - P) to
@@ -474,11 +506,11 @@ __device__ __forceinline__ void run_tile(const BatchArgs& a, Stream<AT>& st, con
         }
 }
 
-// FP64 fits 168 VGPRs: three workgroups per CU (three fit in LDS); FP16/BF16 need more
-// registers, and two fit in LDS.
+// Two workgroups per CU: the 128 x 128 FP64 tile holds 128 accumulator VGPRs per lane,
+// and FP16/BF16 staging takes 60 KB of LDS.
 template <class AT, bool KN>
 __global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(sizeof(typename AT::in_t) == 8 ? 3 : 2)))
+__attribute__((amdgpu_waves_per_eu(2)))
 batched_persistent_kernel(BatchArgs args) {
     using SG = StreamGeom<AT>;
     using VS = Variants<AT>;
@@ -512,8 +544,14 @@ batched_persistent_kernel(BatchArgs args) {
             case 0: run_tile<AT, KN, typename VS::V0>(args, st, q, smem); break;
             case 1: run_tile<AT, KN, typename VS::V1>(args, st, q, smem); break;
             case 2: run_tile<AT, KN, typename VS::V2>(args, st, q, smem); break;
-            default:
-                if constexpr (SG::TMAX >= 128) run_tile<AT, KN, typename VS::V3>(args, st, q, smem);
+            case 3: run_tile<AT, KN, typename VS::V3>(args, st, q, smem); break;
+            default:  // 4 and 5 exist where B is staged 128 columns wide
+                if constexpr (SG::TNMAX >= 128) {
+                    if (st.cur.variant == 4)
+                        run_tile<AT, KN, typename VS::V4>(args, st, q, smem);
+                    else
+                        run_tile<AT, KN, typename VS::V5>(args, st, q, smem);
+                }
                 break;
         }
         // ---- tile boundary ----
@@ -570,11 +608,19 @@ static int bucket_of(const kami_problem& p) {
 
 // Prior variant of a bucket: the largest tile that the problem fills, following the
 // reference's choice of fewer, larger warps for larger sizes
-// (scripts/fp64_batched_GH200.sh:7-14).
+// (scripts/fp64_batched_GH200.sh:7-14).  FP64 takes 128 rows or columns wherever the
+// problem has more than 64 of them (its batches are bound by HBM, and the wider tile
+// halves the re-reads of the other operand).
 static int prior_variant(int prec, int bucket) {
     const int cm = bucket / 25, cn = (bucket / 5) % 5;
     if (cm == 0 && cn == 0) return 1;
-    if (cm >= 3 && cn >= 2 && variants_of(prec) > 3) return 3;
+    if (prec == KAMI_PREC_F64) {
+        if (cm >= 2 && cn >= 2) return 4;
+        if (cm >= 2) return 3;
+        if (cn >= 2) return 5;
+        return 0;
+    }
+    if (cm >= 3 && cn >= 2) return 3;
     return 0;
 }
 
@@ -733,8 +779,8 @@ static kami_status check_problem(kami_problem& p, int b_layout, int ei, int eo) 
     // (stream_load): the farthest piece of any tile must end below kBufferRange.
     const int64_t a_span = ((int64_t)(std::min(p.m, 128) - 1) * p.lda + p.k) * ei;
     const int64_t b_span = b_layout == KAMI_B_NK
-                               ? ((int64_t)(std::min(p.n, 64) - 1) * p.ldb + p.k) * ei
-                               : ((int64_t)(p.k - 1) * p.ldb + std::min(p.n, 64)) * ei;
+                               ? ((int64_t)(std::min(p.n, 128) - 1) * p.ldb + p.k) * ei
+                               : ((int64_t)(p.k - 1) * p.ldb + std::min(p.n, 128)) * ei;
     if (a_span > kBufferRange || b_span > kBufferRange) return KAMI_ENOTSUP;
     return KAMI_OK;
 }

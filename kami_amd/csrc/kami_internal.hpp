@@ -26,7 +26,7 @@ std::vector<TunedBlock> tuned_block_choices();
 void clear_tuned_block_choices();
 // Batched tile variant recorded per (precision, size bucket); -1 when none.
 constexpr int kBatchedBuckets = 125;
-constexpr int kBatchedVariants = 4;
+constexpr int kBatchedVariants = 6;
 int batched_tuned_variant(int prec, int bucket);
 void set_batched_tuned_variant(int prec, int bucket, int variant);
 void clear_batched_tuned_variants();

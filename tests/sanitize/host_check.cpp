@@ -420,7 +420,7 @@ void check_batched_arguments() {
     for (kami_problem& p : big) p.m = p.n = p.k = 16;
     CHECK(kami_batched_gemm_static(big.data(), kStaticItems + 1, 0, &o, nullptr) == KAMI_ENOSPC,
           "static too many problems");
-    for (kami_problem& p : big) p.m = p.n = 128;
+    for (kami_problem& p : big) p.m = p.n = 256;  // four tiles or more each
     CHECK(kami_batched_gemm_static(big.data(), kStaticItems, 0, &o, nullptr) == KAMI_ENOSPC,
           "static too many tiles");
     if (!g_have_device) {

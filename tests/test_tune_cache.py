@@ -59,7 +59,7 @@ def test_save_round_trip(tmp_path):
     "kami-tune 1\nblock 0 64 64 64 3 7 1\n",         # no such wave grid
     "kami-tune 1\nblock 3 128 128 128 2 4 1\n",      # split precision has no tuned partition
     "kami-tune 1\nbatched 0 125 0\n",                # bucket out of range
-    "kami-tune 1\nbatched 0 3 4\n",                  # variant out of range
+    "kami-tune 1\nbatched 0 3 6\n",                  # variant out of range
     "kami-tune 1\nbatched 3 3 1\n",                  # batched has no split precision
     "kami-tune 1\nsomething else\n",
 ])

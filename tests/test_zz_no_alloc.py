@@ -42,12 +42,12 @@ def call_static(arr, count, prec, b_layout=_abi.B_NK, alpha=1.0, beta=0.0, autot
 
 
 def test_limits_and_options_checked_on_the_host():
-    # more problems than queue entries, and more entries (four 64 x 64 tiles per
-    # 128 x 128 FP64 problem) than the array holds: refused before any device call
+    # more problems than queue entries, and more entries (four 128 x 128 tiles per
+    # 256 x 256 FP64 problem) than the array holds: refused before any device call
     arr = problem_array([(16, 16, 16)] * (MAX_ENTRIES + 1))
     assert call_static(arr, MAX_ENTRIES + 1, K.PREC_F64) == _abi.ENOSPC
     n = MAX_ENTRIES // 4 + 1
-    arr = problem_array([(128, 128, 128)] * n)
+    arr = problem_array([(256, 256, 16)] * n)
     assert call_static(arr, n, K.PREC_F64) == _abi.ENOSPC
     # autotuning and graphs need memory of their own
     arr = problem_array([(64, 64, 64)])
