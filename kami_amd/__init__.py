@@ -98,10 +98,17 @@ def _out_dtype(prec):
     return torch.float32 if prec in (PREC_F16_F32, PREC_BF16_F32) else torch.float64
 
 
+def _on_gpu(t, name):
+    """Raise ValueError unless tensor t lives in device memory: the library hands its
+    pointer to a kernel, and a host pointer there faults the GPU."""
+    if t.device.type != "cuda":
+        raise ValueError(f"{name} must be a GPU tensor, got one on {t.device}")
+
+
 def _expect(t, name, shape, dtype, device):
-    """Raise ValueError unless tensor t has exactly this shape, dtype and device and is
-    contiguous: the library receives raw pointers and would read or write past a tensor
-    that does not match."""
+    """Raise ValueError unless tensor t has exactly this shape, dtype and device, is
+    contiguous and lives on a GPU: the library receives raw device pointers and would
+    read or write past a tensor that does not match."""
     if tuple(t.shape) != tuple(shape):
         raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
     if t.dtype != dtype:
@@ -110,6 +117,7 @@ def _expect(t, name, shape, dtype, device):
         raise ValueError(f"{name} is on {t.device}, expected {device}")
     if not t.is_contiguous():
         raise ValueError(f"{name} must be contiguous")
+    _on_gpu(t, name)
 
 
 def _stream_ptr(stream):
@@ -197,6 +205,7 @@ def split_slices(X, *, precision=PREC_F64_SPLIT_F16, slices=6, stream=None):
     import torch
     if X.dim() != 2 or X.dtype != torch.float64 or X.stride(1) != 1:
         raise ValueError("split_slices expects a (rows, k) float64 tensor with unit column stride")
+    _on_gpu(X, "X")
     rows, k = X.shape
     digits = torch.empty((slices, rows, k), dtype=torch.int16, device=X.device)
     exps = torch.empty((rows, (k + 31) // 32), dtype=torch.int32, device=X.device)
@@ -217,6 +226,7 @@ def generate(kind, count, *, precision=PREC_F64, offset=0, device="cuda", stream
     if dt is None:
         raise KamiError(_abi.ENOTSUP, f"generate: precision {precision}")
     out = torch.empty(count, dtype=dt, device=device)
+    _on_gpu(out, "generate's output")
     check(lib().kami_generate(kind, precision, ctypes.c_void_p(out.data_ptr()), count, offset,
                               _stream_ptr(stream)), f"kami_generate(kind={kind})")
     return out
@@ -344,6 +354,7 @@ class BsrMatrix:
                 raise ValueError(f"{name}: rowptr, colidx and vals must be contiguous")
             if t.device != self.rowptr.device:
                 raise ValueError(f"{name}: rowptr, colidx and vals must be on one device")
+        _on_gpu(self.rowptr, f"{name}: rowptr")
 
     def _c(self):
         """The C descriptor (kami_bsr), rebuilt only when the arrays change."""
@@ -502,6 +513,7 @@ def bsr_spmm(A: BsrMatrix, B, C=None, *, b_layout="nk", precision=None, alpha=1.
         raise ValueError(f"C must be {out_dt} of shape {(A.mb * A.bs, n)} with unit column stride")
     if C.device != B.device or A.rowptr.device != B.device:
         raise ValueError("A, B and C must be on one device")
+    _on_gpu(B, "B")
     a = A._c()
     check(lib().kami_bsr_spmm(ctypes.byref(a), ctypes.c_void_p(B.data_ptr()), n, B.stride(0),
                               B_KN if kn else B_NK, ctypes.c_void_p(C.data_ptr()), C.stride(0),

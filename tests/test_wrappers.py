@@ -103,3 +103,18 @@ def test_spgemm_rejects_mismatch():
     with pytest.raises(ValueError):
         K.spgemm_numeric(A, bsr(4, 4, 32, 1, dtype=torch.float64),
                          torch.zeros(5, dtype=torch.int32), torch.zeros(1, dtype=torch.int32))
+
+
+def test_host_tensors_are_refused_before_any_library_call():
+    """Operands that match in every other way but live in host memory: the library would
+    hand their pointers to a kernel, so the wrappers refuse them."""
+    with pytest.raises(ValueError, match="GPU tensor"):
+        K.block_gemm(t(2, 64, 64), t(2, 64, 64), t(2, 64, 64))
+    with pytest.raises(ValueError, match="GPU tensor"):
+        K.split_gemm(t(2, 128, 128), t(2, 128, 128))
+    with pytest.raises(ValueError, match="GPU tensor"):
+        K.split_slices(t(4, 64))
+    with pytest.raises(ValueError, match="GPU tensor"):
+        K.generate(K.GEN_RAMP, 16, device="cpu")
+    with pytest.raises(ValueError, match="GPU tensor"):
+        K.BatchedPlan([(t(32, 16), t(48, 16), t(32, 48))])
