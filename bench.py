@@ -332,6 +332,7 @@ def main():
         for name, fn in [("w1_f64_64cubed_x1024", W.run_w1), ("w2_f16_block_gemm", W.run_w2),
                          ("w3r_resident_f64_64cubed_x4096", W.run_w3r),
                          ("w4_batched_uneven_x20000", W.run_w4), ("w5_block_spgemm", W.run_w5),
+                         ("w5r_spgemm_reference_measurement", W.run_w5r),
                          ("w6_block_spmm", W.run_w6)]:
             t0 = time.perf_counter()
             try:

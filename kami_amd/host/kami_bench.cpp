@@ -371,21 +371,45 @@ int run_spgemm(const Args& a) {
     const float sparsity = (float)a.getd("--sparsity", 0.5);
     // NUM_ITER: the numeric phase repeats its block products in the kernel
     const int iters = (int)a.geti("--iters", 1);
+    // the reference's measurement runs the same small product in each of its 16384 CTAs
+    // (BLOCKS_PER_GRID, src/sparse/spgemm/1d/block_spgemm_1d_half_mma.cu:43); here that
+    // is one block-diagonal matrix of `copies` copies, each copy's product independent
+    const int copies = (int)a.geti("--copies", 1);
+    if (copies < 1 || (long)copies * nbr > (1L << 24)) {
+        std::fprintf(stderr, "spgemm: --copies from 1 to 2^24 / blocks\n");
+        return 2;
+    }
     print_device();
-    std::vector<int32_t> rp, ci;
-    std::vector<double> val;
-    const int nnzb = gen_bsr(nbr, nbr, bs, sparsity, rp, ci, val);  // A = B, as srand(0) twice
-    std::printf("nnzA = %d, nnzB = %d (blocks)\n", nnzb, nnzb);
+    std::vector<int32_t> rp1, ci1;
+    std::vector<double> val1;
+    const int nnzb1 = gen_bsr(nbr, nbr, bs, sparsity, rp1, ci1, val1);  // A = B, as srand(0) twice
+    std::printf("nnzA = %d, nnzB = %d (blocks)\n", nnzb1, nnzb1);
+    const int mb = nbr * copies;
+    const long nnzb_long = (long)nnzb1 * copies;
+    if (nnzb_long >= (1L << 31)) {
+        std::fprintf(stderr, "spgemm: too many blocks\n");
+        return 2;
+    }
+    const int nnzb = (int)nnzb_long;
+    std::vector<int32_t> rp(mb + 1), ci((size_t)nnzb);
+    std::vector<double> val((size_t)nnzb * bs * bs);
+    for (int c = 0; c < copies; ++c) {
+        for (int i = 0; i < nbr; ++i) rp[(size_t)c * nbr + i] = c * nnzb1 + rp1[i];
+        for (int q = 0; q < nnzb1; ++q) ci[(size_t)c * nnzb1 + q] = c * nbr + ci1[q];
+        std::copy(val1.begin(), val1.end(), val.begin() + (size_t)c * val1.size());
+    }
+    rp[mb] = nnzb;
+    if (copies > 1) std::printf("copies = %d (block-diagonal, %d block rows)\n", copies, mb);
     int32_t *drp = nullptr, *dci = nullptr;
     HIP_OK(hipMalloc(&drp, rp.size() * 4));
     HIP_OK(hipMalloc(&dci, std::max<size_t>(ci.size(), 1) * 4));
     HIP_OK(hipMemcpy(drp, rp.data(), rp.size() * 4, hipMemcpyHostToDevice));
     if (!ci.empty()) HIP_OK(hipMemcpy(dci, ci.data(), ci.size() * 4, hipMemcpyHostToDevice));
     void* dval = upload(val.empty() ? std::vector<double>(1, 0.0) : val, prec);
-    kami_bsr A{nbr, nbr, bs, nnzb, drp, dci, dval};
+    kami_bsr A{mb, mb, bs, nnzb, drp, dci, dval};
 
     int32_t* drpC = nullptr;
-    HIP_OK(hipMalloc(&drpC, (nbr + 1) * 4));
+    HIP_OK(hipMalloc(&drpC, ((size_t)mb + 1) * 4));
     hipEvent_t t0, t1, t2;
     HIP_OK(hipEventCreate(&t0));
     HIP_OK(hipEventCreate(&t1));
@@ -416,29 +440,38 @@ int run_spgemm(const Args& a) {
     HIP_OK(hipEventElapsedTime(&ms, t1, t2));
     long long nnzCub = 0;  // sum over stored blocks (i, k) of A of the blocks in B's row k
     for (int i = 0; i < nbr; ++i)
-        for (int q = rp[i]; q < rp[i + 1]; ++q) nnzCub += rp[ci[q] + 1] - rp[ci[q]];
-    const double ops = 2.0 * (double)nnzCub * bs * bs * bs * iters;
+        for (int q = rp1[i]; q < rp1[i + 1]; ++q) nnzCub += rp1[ci1[q] + 1] - rp1[ci1[q]];
+    // the reference's count: per matrix, times its CTAs and NUM_ITER (:512-514)
+    const double ops = 2.0 * (double)nnzCub * bs * bs * bs * copies * iters;
     const double gflops = ops / (ms * 1e6), tflops = gflops / 1e3;
     std::printf("Execution time: %g ms\n", ms);
     std::printf("Performance: %g GFLOPS (%g TFLOPS)\n", gflops, tflops);
     std::printf("[hemeng_log],2d,%lld,%d,%d,%d,%d,%g,%d\n", nnzCub, nbr * bs, nbr * bs, nbr * bs,
                 1, tflops, 64);
 
-    // check: structure against a boolean product, values against a dense block product
-    std::vector<int32_t> rpC(nbr + 1), ciC(nnzbC);
-    HIP_OK(hipMemcpy(rpC.data(), drpC, (nbr + 1) * 4, hipMemcpyDeviceToHost));
+    // check: structure against a boolean product, values against a dense block product,
+    // on the first and the last copy (every copy is the same product)
+    std::vector<int32_t> rpC((size_t)mb + 1), ciC(nnzbC);
+    HIP_OK(hipMemcpy(rpC.data(), drpC, ((size_t)mb + 1) * 4, hipMemcpyDeviceToHost));
     if (nnzbC) HIP_OK(hipMemcpy(ciC.data(), dciC, nnzbC * 4, hipMemcpyDeviceToHost));
     std::vector<double> C = download_c(dvalC, (size_t)nnzbC * bs * bs, prec);
     int error = 0;
     std::vector<double> acc((size_t)nbr * bs * bs);
     std::vector<char> hit(nbr);
-    for (int i = 0; i < nbr && !error; ++i) {
+    std::vector<int> check_rows;
+    for (int c : {0, copies - 1}) {
+        if (c == copies - 1 && c == 0 && !check_rows.empty()) break;
+        for (int i = 0; i < nbr; ++i) check_rows.push_back(c * nbr + i);
+    }
+    for (int i : check_rows) {
+        if (error) break;
+        const int c0 = (i / nbr) * nbr;  // first block column of the row's copy
         std::fill(acc.begin(), acc.end(), 0.0);
         std::fill(hit.begin(), hit.end(), 0);
         for (int q = rp[i]; q < rp[i + 1]; ++q) {
             const int kb = ci[q];
             for (int r = rp[kb]; r < rp[kb + 1]; ++r) {
-                const int j = ci[r];
+                const int j = ci[r] - c0;  // column within the copy
                 hit[j] = 1;
                 const double* a_ = &val[(size_t)q * bs * bs];
                 const double* b_ = &val[(size_t)r * bs * bs];
@@ -451,7 +484,7 @@ int run_spgemm(const Args& a) {
         int pos = rpC[i];
         for (int j = 0; j < nbr && !error; ++j) {
             if (!hit[j]) continue;
-            if (pos >= rpC[i + 1] || ciC[pos] != j) {
+            if (pos >= rpC[i + 1] || ciC[pos] != j + c0) {
                 std::printf("Error: structure of block row %d\n", i);
                 error = 1;
                 break;
@@ -574,7 +607,7 @@ void usage() {
         "       kami_bench batched [--prec f64|f16] [--m M --n N --k K] [--count C]\n"
         "                          [--iters I] [--graph 0|1] [--autotune 0|1]\n"
         "       kami_bench spgemm  [--prec f16|bf16|f64] [--blocks B] [--bs 16|32]\n"
-        "                          [--sparsity S] [--iters I]\n"
+        "                          [--sparsity S] [--iters I] [--copies N]\n"
         "       kami_bench spmm    [--prec f16|bf16|f64] [--blocks B] [--bs 16|32] [--n N]\n"
         "                          [--sparsity S]\n");
 }

@@ -5,7 +5,7 @@
     python -m kami_amd.profiling w3 --path native        # FP64 128^3 x 4096 native
     python -m kami_amd.profiling w3 --path split         # same through FP16 slices
     python -m kami_amd.profiling w3 --path split_i8      # same through int8 digits
-    python -m kami_amd.profiling w1 | w3r | w4 --prec f16 | w5 | w6
+    python -m kami_amd.profiling w1 | w3r | w4 --prec f16 | w5 | w5r | w6
     python -m kami_amd.profiling w2 --shape 64x64x64 --part 3d --iters 200  # FP16 resident repeat
 
 Each run launches the kernel `--reps` times after one warm-up, so per-dispatch counters
@@ -23,7 +23,7 @@ import numpy as np
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["copy", "add", "w1", "w2", "w3", "w3r", "w4", "w5", "w6"])
+    ap.add_argument("what", choices=["copy", "add", "w1", "w2", "w3", "w3r", "w4", "w5", "w5r", "w6"])
     ap.add_argument("--shape", default="64x64x64")
     ap.add_argument("--part", choices=["1d", "2d", "3d"], default="2d")
     ap.add_argument("--iters", type=int, default=200)
@@ -98,6 +98,8 @@ def main(argv=None):
         W.run_w4(steps=args.reps, warmup=1, precisions=(args.prec,))
     elif args.what == "w5":
         W.run_w5(steps=args.reps, warmup=1)
+    elif args.what == "w5r":
+        print(W.run_w5r(steps=args.reps, warmup=1))
     elif args.what == "w6":
         W.run_w6(steps=args.reps, warmup=1)
     torch.cuda.synchronize()

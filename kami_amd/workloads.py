@@ -280,6 +280,76 @@ def run_w5(steps=10, warmup=2):
             "numeric_hbm_gbs": byts / ms_num / 1e6}
 
 
+# ------------------------------------------------------------------ w5r
+
+def reference_bsr(blocks=8, bs=16, sparsity=0.5):
+    """The reference programs' BSR input (src/sparse/spgemm/1d/block_spgemm_1d_half_mma.cu,
+    the generator at :372-397): srand(0), then for every block (i, j) in row order a
+    block is stored when (float) rand() / (float) RAND_MAX > sparsity, and its bs x bs
+    values are rand() % 3 + 1, drawn right after.  The C library's own rand() is called
+    through ctypes, so the pattern is the reference's on this host's glibc."""
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    libc.rand.restype = ctypes.c_int
+    libc.srand(0)
+    rand_max = np.float32(2147483647)
+    rp, ci, vals = [0], [], []
+    for _ in range(blocks):
+        for j in range(blocks):
+            if np.float32(libc.rand()) / rand_max > np.float32(sparsity):
+                ci.append(j)
+                vals.append([libc.rand() % 3 + 1 for _ in range(bs * bs)])
+        rp.append(len(ci))
+    return (np.array(rp, np.int32), np.array(ci, np.int32),
+            np.array(vals, np.float64).reshape(-1, bs, bs))
+
+
+def run_w5r(copies=16384, iters=1000, blocks=8, bs=16, steps=3, warmup=1):
+    """The reference's SpGEMM measurement: every one of its 16384 CTAs multiplies the same
+    BSR matrix by itself NUM_ITER = 1000 times inside the kernel and the FLOP count is
+    2 nnzCub bs^3 x 16384 x 1000 (src/sparse/spgemm/1d/block_spgemm_1d_half_mma.cu:39,43,
+    512-514).  Here the matrix (the reference's default: 8 x 8 blocks of 16 at 50 %
+    density, its generator) is laid out `copies` times along the diagonal of one
+    block-diagonal BSR matrix, whose product is the copies' products; the numeric phase
+    repeats each block product `iters` times on register-held operands
+    (kami_bsr_spgemm_numeric_repeat)."""
+    torch = _torch()
+    dev = torch.device("cuda:0")
+    rp1, ci1, v1 = reference_bsr(blocks, bs)
+    nnz1 = len(ci1)
+    nnzcub1 = int(sum(int(rp1[c + 1] - rp1[c]) for c in ci1))
+    mb = blocks * copies
+    rp = np.concatenate([rp1[:-1] + c * nnz1 for c in range(copies)] + [[nnz1 * copies]]).astype(np.int32)
+    ci = (np.tile(ci1, copies) + np.repeat(np.arange(copies, dtype=np.int32) * blocks, nnz1)).astype(np.int32)
+    vals = torch.from_numpy(v1.astype(np.float16)).to(dev).repeat(copies, 1, 1)
+    A = K.BsrMatrix(mb, mb, bs, torch.from_numpy(rp).to(dev), torch.from_numpy(ci).to(dev), vals)
+    rowptr, nnz = K.spgemm_symbolic(A, A)
+    colidx = K.spgemm_fill(A, A, rowptr, nnz)
+    out = {}
+    ms = time_ms(lambda: out.__setitem__("C", K.spgemm_numeric(A, A, rowptr, colidx, iters=iters)),
+                 steps, warmup)
+    # the first copy against its product in FP64 (integer values: exact)
+    Cv = out["C"][: int(rowptr[blocks].item())].double().cpu().numpy() / iters
+    rpc = rowptr[: blocks + 1].cpu().numpy()
+    cic = colidx[: int(rpc[-1])].cpu().numpy()
+    err = 0.0
+    for i in range(blocks):
+        for q in range(rpc[i], rpc[i + 1]):
+            ref = np.zeros((bs, bs))
+            for a in range(rp1[i], rp1[i + 1]):
+                k = ci1[a]
+                for b in range(rp1[k], rp1[k + 1]):
+                    if ci1[b] == cic[q]:
+                        ref += v1[a] @ v1[b]
+            err = max(err, float(np.max(np.abs(Cv[q] - ref))))
+    flops = 2.0 * nnzcub1 * bs ** 3 * copies * iters
+    return {"workload": f"reference SpGEMM measurement: {blocks * bs}^2 BSR ({blocks}x{blocks} blocks of "
+                        f"{bs}, 50% density, the reference's generator) x {copies} copies x {iters} "
+                        "in-kernel repetitions, FP16->FP32",
+            "nnzCub_per_matrix": nnzcub1, "nnzb_C": int(nnz), "ms": ms,
+            "tflops": flops / ms / 1e9, "max_abs_err_first_copy": err}
+
+
 # ------------------------------------------------------------------ w6
 
 def run_w6(steps=20, warmup=3, n=256):

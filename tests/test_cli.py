@@ -48,6 +48,9 @@ CASES = [
     (["spgemm", "--prec", "f16", "--blocks", "8", "--bs", "16"], r"\[hemeng_log\],2d,\d+,128,128,128,"),
     (["spgemm", "--prec", "f16", "--blocks", "8", "--bs", "16", "--iters", "1000"],
      r"\[hemeng_log\],2d,\d+,128,128,128,"),
+    # the reference's measurement layout: the same matrix many times (block-diagonal)
+    (["spgemm", "--prec", "f16", "--blocks", "8", "--bs", "16", "--iters", "10", "--copies", "512"],
+     r"\[hemeng_log\],2d,149,128,128,128,"),
     (["spgemm", "--prec", "f64", "--blocks", "24", "--bs", "32", "--sparsity", "0.8"],
      r"\[hemeng_log\],2d,\d+,768,768,768,"),
     (["spmm", "--prec", "f16", "--blocks", "8", "--bs", "16"], r"\[hemeng_log\],2d,128,4,"),
