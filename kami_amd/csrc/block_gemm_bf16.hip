@@ -1,4 +1,5 @@
-// Block-GEMM instances: BF16 inputs, FP32 accumulation (16x16x32 and 16x16x16 bf16 MFMA).
+// Block-GEMM instances: BF16 inputs, FP32 accumulation (16x16x32, 16x16x16 and 32x32x16 bf16
+// MFMA).
 // Shapes are the reference's FP16 sweep (scripts/fp16_block_square_GH200.sh:11-25,
 // scripts/fp16_block_lowrank_GH200.sh:20-27): squares {16, 32, 64, 128, 192, 256} and
 // low-rank M = N in {32, 64, 128, 192, 256} with K in {16, 32}.  The 3D partition at
@@ -12,6 +13,7 @@ namespace kami {
 void register_bf16_instances(std::vector<BlockInstance>& v) {
     using A = AtomBF16x32;
     using H = AtomBF16x16;  // K step 16, for K slices that are not multiples of 32
+    using W = AtomBF16M32;  // 32x32x16: full rate at K = 16, wave tiles multiples of 32
     using H8 = AtomBF16x16K8;  // K step 8: the 3D split of K = 16
     constexpr int F = 2;
     // 32^3
@@ -43,9 +45,9 @@ void register_bf16_instances(std::vector<BlockInstance>& v) {
     add_resident_instance<A, F, 256, 256, 256, 16>(v, 16);
     add_instance<A, F, 256, 256, 256, 4, 4, 1>(v, 2, 4);
     // low rank 256 x 256 x 16 and x 32
-    add_instance<H, F, 256, 256, 16, 8, 1, 1>(v, 1, 8);
+    add_instance<W, F, 256, 256, 16, 8, 1, 1>(v, 1, 8);
     add_instance<H, F, 256, 256, 16, 16, 1, 1>(v, 1, 16);
-    add_instance<H, F, 256, 256, 16, 4, 4, 1>(v, 2, 4);
+    add_instance<W, F, 256, 256, 16, 4, 4, 1>(v, 2, 4);
     add_instance<A, F, 256, 256, 32, 8, 1, 1>(v, 1, 8);
     add_instance<A, F, 256, 256, 32, 16, 1, 1>(v, 1, 16);
     add_instance<A, F, 256, 256, 32, 4, 4, 1>(v, 2, 4);
@@ -74,9 +76,9 @@ void register_bf16_instances(std::vector<BlockInstance>& v) {
     add_instance<A, F, 64, 64, 32, 4, 1, 1>(v, 1, 4);
     add_instance<A, F, 64, 64, 32, 2, 2, 1>(v, 2, 2);
     add_instance<H, F, 128, 128, 16, 2, 1, 1>(v, 1, 2);
-    add_instance<H, F, 128, 128, 16, 4, 1, 1>(v, 1, 4);
+    add_instance<W, F, 128, 128, 16, 4, 1, 1>(v, 1, 4);
     add_instance<H, F, 128, 128, 16, 8, 1, 1>(v, 1, 8);
-    add_instance<H, F, 128, 128, 16, 2, 2, 1>(v, 2, 2);
+    add_instance<W, F, 128, 128, 16, 2, 2, 1>(v, 2, 2);
     add_instance<A, F, 128, 128, 32, 2, 1, 1>(v, 1, 2);
     add_instance<A, F, 128, 128, 32, 4, 1, 1>(v, 1, 4);
     add_instance<A, F, 128, 128, 32, 8, 1, 1>(v, 1, 8);

@@ -36,7 +36,9 @@ struct BlockGeom {
     static constexpr int WAVES = PM * PN * PK;
     static constexpr int NT = WAVES * kWave;
     static constexpr int TM = BM / PM, TN = BN / PN;
-    static constexpr int IM = TM / 16, JN = TN / 16;
+    // atom edge (16, or 32 for the 32 x 32 atoms) and accumulators per lane
+    static constexpr int DIM = AtomShape<AT>::DIM, NACC = AtomShape<AT>::NACC;
+    static constexpr int IM = TM / DIM, JN = TN / DIM;
     static constexpr int NS = BK / KS;
     static constexpr int KSW = KS / PK;
     static constexpr int KSTEPS = KSW / AT::KSTEP;
@@ -50,7 +52,7 @@ struct BlockGeom {
     static constexpr int STAGE_LDS = A_LDS + B_LDS;
     static constexpr int STAGING = NBUF * STAGE_LDS;
     static constexpr int ACC_ELT = (int)sizeof(typename AT::acc_t);
-    static constexpr int RED_ROW_BYTES = PK * PM * PN * JN * 256 * ACC_ELT;
+    static constexpr int RED_ROW_BYTES = PK * PM * PN * JN * NACC * 64 * ACC_ELT;
     static constexpr int RED_ROWS = (PK > 1) ? cmin(IM, cmax(1, STAGING / RED_ROW_BYTES)) : 1;
     static constexpr int RED_ROUNDS = (IM + RED_ROWS - 1) / RED_ROWS;
     static constexpr int RED_LDS = (PK > 1) ? RED_ROWS * RED_ROW_BYTES : 0;
@@ -61,7 +63,7 @@ struct BlockGeom {
     static constexpr int NBC = (CB + NT - 1) / NT;
 
     static_assert(BM % PM == 0 && BN % PN == 0 && BK % KS == 0 && KS % PK == 0, "tiling");
-    static_assert(TM % 16 == 0 && TN % 16 == 0, "wave tile must be a multiple of 16");
+    static_assert(TM % DIM == 0 && TN % DIM == 0, "wave tile must be a multiple of the atom");
     static_assert(KSW % AT::KSTEP == 0, "wave K slice must be a multiple of the atom K step");
     static_assert((KS * ELT) % 16 == 0 && (BN * ELT) % 16 == 0, "16-byte staging");
     static_assert(NT <= 1024, "workgroup too large");
@@ -123,7 +125,7 @@ block_gemm_kernel(BlockArgs args) {
 #pragma unroll
         for (int j = 0; j < G::JN; ++j)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) acc[i][j][v] = 0;
+            for (int v = 0; v < G::NACC; ++v) acc[i][j][v] = 0;
 
     // ---- staging of the shared operands (global -> registers -> LDS) ----
     u4_t stA[G::NA > 0 ? G::NA : 1];
@@ -211,10 +213,10 @@ block_gemm_kernel(BlockArgs args) {
     frag_t a_nxt[(G::A_PRIV && G::NS > 1) ? AIM : 1][(G::A_PRIV && G::NS > 1) ? AKS : 1];
     auto load_a = [&](int s, frag_t (&dst)[AIM][AKS]) {
         if constexpr (G::A_PRIV) {
-            const int kb = s * G::KS + pk * G::KSW + AT::KV * (lane >> 4);
+            const int kb = s * G::KS + pk * G::KSW + AT::KV * (lane / G::DIM);
 #pragma unroll
             for (int im = 0; im < G::IM; ++im) {
-                const in_t* row = A + (int64_t)(pm * G::TM + im * 16 + (lane & 15)) * lda + kb;
+                const in_t* row = A + (int64_t)(pm * G::TM + im * G::DIM + (lane % G::DIM)) * lda + kb;
 #pragma unroll
                 for (int ks = 0; ks < G::KSTEPS; ++ks)
                     dst[im][ks] = bitcast<frag_t>(load_raw<FB>(row + ks * AT::KSTEP));
@@ -225,7 +227,7 @@ block_gemm_kernel(BlockArgs args) {
     auto compute = [&](int buf) {
         const unsigned char* base = smem + buf * G::STAGE_LDS;
         const unsigned char* bb = base + G::A_LDS;
-        const int kin = pk * G::KSW + AT::KV * (lane >> 4);
+        const int kin = pk * G::KSW + AT::KV * (lane / G::DIM);
 #pragma unroll
         for (int ks = 0; ks < G::KSTEPS; ++ks) {
             const int kk = (kin + ks * AT::KSTEP) * G::ELT;
@@ -238,14 +240,14 @@ block_gemm_kernel(BlockArgs args) {
                     af[im] = a_cur[im][ks];
                 } else {
                     af[im] = bitcast<frag_t>(
-                        load_raw<FB>(base + (pm * G::TM + im * 16 + (lane & 15)) * G::ROWB + kk));
+                        load_raw<FB>(base + (pm * G::TM + im * G::DIM + (lane % G::DIM)) * G::ROWB + kk));
                 }
             }
             if constexpr (AT::NSUB == 1) {
 #pragma unroll
                 for (int jn = 0; jn < G::JN; ++jn) {
                     const frag_t bf = bitcast<frag_t>(
-                        load_raw<FB>(bb + (pn * G::TN + jn * 16 + (lane & 15)) * G::ROWB + kk));
+                        load_raw<FB>(bb + (pn * G::TN + jn * G::DIM + (lane % G::DIM)) * G::ROWB + kk));
 #pragma unroll
                     for (int im = 0; im < G::IM; ++im) AT::mma(acc[im][jn], af[im], bf);
                 }
@@ -256,7 +258,7 @@ block_gemm_kernel(BlockArgs args) {
 #pragma unroll
                 for (int jn = 0; jn < G::JN; ++jn)
                     bf[jn] = bitcast<frag_t>(
-                        load_raw<FB>(bb + (pn * G::TN + jn * 16 + (lane & 15)) * G::ROWB + kk));
+                        load_raw<FB>(bb + (pn * G::TN + jn * G::DIM + (lane % G::DIM)) * G::ROWB + kk));
 #pragma unroll
                 for (int e = 0; e < AT::NSUB; ++e)
 #pragma unroll
@@ -318,7 +320,8 @@ block_gemm_kernel(BlockArgs args) {
                             (((pk * G::PM + pm) * G::PN + pn) * G::RED_ROWS + (im - r * G::RED_ROWS)) *
                                 G::JN + jn;
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) red[slot * 256 + v * 64 + lane] = acc[im][jn][v];
+                        for (int v = 0; v < G::NACC; ++v)
+                            red[slot * (G::NACC * 64) + v * 64 + lane] = acc[im][jn][v];
                     }
                 }
             __syncthreads();
@@ -338,7 +341,8 @@ block_gemm_kernel(BlockArgs args) {
                                     (((q * G::PM + pm) * G::PN + pn) * G::RED_ROWS +
                                      (im - r * G::RED_ROWS)) * G::JN + jn;
 #pragma unroll
-                                for (int v = 0; v < 4; ++v) x[v] = red[slot * 256 + v * 64 + lane];
+                                for (int v = 0; v < G::NACC; ++v)
+                                    x[v] = red[slot * (G::NACC * 64) + v * 64 + lane];
                             }
                             if (q == 0) tot = x; else tot += x;
                         }
@@ -356,10 +360,10 @@ block_gemm_kernel(BlockArgs args) {
 #pragma unroll
         for (int jn = 0; jn < G::JN; ++jn) {
             if (G::PK > 1 && (im * G::JN + jn) % G::PK != pk) continue;
-            const int col = pn * G::TN + jn * 16 + (lane & 15);
+            const int col = pn * G::TN + jn * G::DIM + (lane % G::DIM);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = pm * G::TM + im * 16 + AT::out_row(lane, v);
+            for (int v = 0; v < G::NACC; ++v) {
+                const int row = pm * G::TM + im * G::DIM + AT::out_row(lane, v);
                 out_t* dst = C + (int64_t)row * ldc + col;
                 acc_t val = alpha * acc[im][jn][v];
                 if (beta != (acc_t)0) val += beta * (acc_t)(*dst);

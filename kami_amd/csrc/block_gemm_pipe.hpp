@@ -20,6 +20,7 @@ template <class AT, class G, int PD>
 __global__ void __launch_bounds__(G::NT)
 block_gemm_pipe_kernel(BlockArgs args, int64_t count) {
     static_assert(G::PK == 1 && !G::A_PRIV, "pipelined form is the 2D partition");
+    static_assert(G::DIM == 16, "pipelined form takes 16 x 16 atoms");
     static_assert(PD >= 1 && G::NS % PD == 0, "prefetch depth must divide the stage count");
     using in_t = typename AT::in_t;
     using acc_t = typename AT::acc_t;

@@ -21,6 +21,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 #include <stdint.h>
 
 namespace kami {
@@ -148,6 +150,65 @@ struct AtomBF16x16 {
         c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
     }
     __device__ static inline int out_row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+typedef float f16x_t __attribute__((ext_vector_type(16)));
+
+// 32 x 32 x 16 atoms (gfx950 MFMA, full rate where 16x16x16 runs at half): a wave computes
+// a 32 x 32 tile; lane l holds column (or A row) l % 32, k values 8 (l / 32) .. + 7, and
+// 16 accumulators, rows 8 (v / 4) + 4 (l / 32) + v % 4 for v < 16.  Only the block-GEMM
+// template takes them (it reads DIM and NACC); 16 x 16 atoms leave both out.
+struct AtomF16M32 {
+    using in_t = uint16_t;
+    using acc_t = float;
+    using out_t = float;
+    using frag_t = h8_t;
+    using accv_t = f16x_t;
+    using raw_t = u4_t;
+    static constexpr int KV = 8;
+    static constexpr int KSTEP = 16;
+    static constexpr int FRAG_BYTES = 16;
+    static constexpr int NSUB = 1;
+    static constexpr int DIM = 32, NACC = 16;
+    __device__ static inline void mma_sub(accv_t& c, const frag_t& a, const frag_t& b, int) {
+        mma(c, a, b);
+    }
+    __device__ static inline void mma(accv_t& c, const frag_t& a, const frag_t& b) {
+        c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline int out_row(int lane, int v) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
+};
+
+struct AtomBF16M32 {
+    using in_t = uint16_t;
+    using acc_t = float;
+    using out_t = float;
+    using frag_t = bf8_t;
+    using accv_t = f16x_t;
+    using raw_t = u4_t;
+    static constexpr int KV = 8;
+    static constexpr int KSTEP = 16;
+    static constexpr int FRAG_BYTES = 16;
+    static constexpr int NSUB = 1;
+    static constexpr int DIM = 32, NACC = 16;
+    __device__ static inline void mma_sub(accv_t& c, const frag_t& a, const frag_t& b, int) {
+        mma(c, a, b);
+    }
+    __device__ static inline void mma(accv_t& c, const frag_t& a, const frag_t& b) {
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline int out_row(int lane, int v) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
+};
+
+// Edge of an atom's output tile and accumulators per lane: 16 and 4 unless the atom
+// names them (the 32 x 32 atoms).
+template <class AT, class = void>
+struct AtomShape {
+    static constexpr int DIM = 16, NACC = 4;
+};
+template <class AT>
+struct AtomShape<AT, std::void_t<decltype(AT::DIM)>> {
+    static constexpr int DIM = AT::DIM, NACC = AT::NACC;
 };
 
 // K step 8 on the 16x16x16 instruction, for the 3D split of K = 16 over two depths: lane
