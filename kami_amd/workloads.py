@@ -104,7 +104,11 @@ W2_SHAPES = [(32, 32, 32), (64, 64, 64), (128, 128, 128), (256, 256, 256), (256,
              (256, 256, 32)]
 
 
-def run_w2(count=8192, steps=10, warmup=2, resident_tflop=4.0):
+def run_w2(count=8192, steps=10, warmup=2, res_count=16384, res_iters=1000):
+    """FP16 block GEMM at the cfg2 shapes, every partition: streaming over `count`
+    problems, and the reference's resident-repeat measurement, `res_count` problems (its
+    BLOCKS_PER_GRID) each accumulating its product `res_iters` times inside the kernel
+    (NUM_ITER; src/block_gemm/KAMI/1d/fp16/block_gemm_1d_half_mma.cu:37,41,130,256)."""
     torch = _torch()
     dev = torch.device("cuda:0")
     out = {}
@@ -117,8 +121,10 @@ def run_w2(count=8192, steps=10, warmup=2, resident_tflop=4.0):
         ref = torch.bmm(A[:16].double(), B[:16].double().transpose(1, 2).contiguous())
         flops = 2.0 * m * n * k * count
         byts = (2.0 * (m * k + n * k) + 4.0 * m * n) * count
-        cres = min(count, 2048)
-        it_res = max(1, int(round(resident_tflop * 1e12 / (2.0 * m * n * k * cres))))
+        reps = (res_count + count - 1) // count  # the resident problems: the inputs repeated
+        Ar = A.repeat(reps, 1, 1)[:res_count]
+        Br = B.repeat(reps, 1, 1)[:res_count]
+        Cr = torch.empty((res_count, m, n), dtype=torch.float32, device=dev)
         shape_out = {}
         for part in ("1d", "2d", "3d"):
             try:
@@ -128,16 +134,15 @@ def run_w2(count=8192, steps=10, warmup=2, resident_tflop=4.0):
                 continue
             ms = time_ms(lambda: K.block_gemm(A, B, C, partition=part), steps, warmup)
             err = float((C[:16].double() - ref).abs().max() / ref.abs().max())
-            msr = time_ms(lambda: K.block_gemm(A[:cres], B[:cres], C[:cres], partition=part,
-                                               iters=it_res), 3, 1)
+            msr = time_ms(lambda: K.block_gemm(Ar, Br, Cr, partition=part, iters=res_iters), 3, 1)
             tf = flops / ms / 1e9
-            tfr = 2.0 * m * n * k * cres * it_res / msr / 1e9
+            tfr = 2.0 * m * n * k * res_count * res_iters / msr / 1e9
             shape_out[part] = {"ranks": cfg["ranks"], "streaming_tflops": tf,
                                "streaming_hbm_gbs": byts / ms / 1e6,
                                "resident_tflops": tfr, "resident_frac_f16_peak": tfr / PEAK["f16"],
                                "max_rel_err_vs_fp64": err}
         out[f"{m}x{n}x{k}"] = shape_out
-        del A, B, C
+        del A, B, C, Ar, Br, Cr
     return out
 
 
