@@ -415,15 +415,27 @@ int run_spgemm(const Args& a) {
     HIP_OK(hipEventCreate(&t1));
     HIP_OK(hipEventCreate(&t2));
     int64_t nnzbC = 0;
-    HIP_OK(hipEventRecord(t0, nullptr));
-    KAMI_OK_OR_DIE(kami_bsr_spgemm_symbolic(&A, &A, drpC, &nnzbC, nullptr, 0, nullptr));
     int32_t* dciC = nullptr;
-    HIP_OK(hipMalloc(&dciC, std::max<int64_t>(nnzbC, 1) * 4));
-    KAMI_OK_OR_DIE(kami_bsr_spgemm_fill(&A, &A, drpC, dciC, nullptr, 0, nullptr));
-    HIP_OK(hipEventRecord(t1, nullptr));
-    HIP_OK(hipEventSynchronize(t1));
+    // the symbolic phase as the reference times it (SPG-1D:285-370): count, scan, the
+    // total read back, C's column array allocated, fill; once untimed (the first call on
+    // a thread allocates the pinned word the total comes back through), then `reps`
+    // times, each allocation freed before the next
+    const int reps = std::max(1, (int)a.geti("--sym-reps", 1));
     float sym_ms = 0;
-    HIP_OK(hipEventElapsedTime(&sym_ms, t0, t1));
+    for (int r = -1; r < reps; ++r) {
+        if (dciC) HIP_OK(hipFree(dciC));
+        dciC = nullptr;
+        HIP_OK(hipDeviceSynchronize());
+        HIP_OK(hipEventRecord(t0, nullptr));
+        KAMI_OK_OR_DIE(kami_bsr_spgemm_symbolic(&A, &A, drpC, &nnzbC, nullptr, 0, nullptr));
+        HIP_OK(hipMalloc(&dciC, std::max<int64_t>(nnzbC, 1) * 4));
+        KAMI_OK_OR_DIE(kami_bsr_spgemm_fill(&A, &A, drpC, dciC, nullptr, 0, nullptr));
+        HIP_OK(hipEventRecord(t1, nullptr));
+        HIP_OK(hipEventSynchronize(t1));
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, t0, t1));
+        if (r >= 0) sym_ms += ms / reps;
+    }
     std::printf("nnzC = %lld (blocks)\n", (long long)nnzbC);
     std::printf("Symbolic time: %g ms\n", sym_ms);
     const size_t eo = half_input(prec) ? 4 : 8;
@@ -607,7 +619,7 @@ void usage() {
         "       kami_bench batched [--prec f64|f16] [--m M --n N --k K] [--count C]\n"
         "                          [--iters I] [--graph 0|1] [--autotune 0|1]\n"
         "       kami_bench spgemm  [--prec f16|bf16|f64] [--blocks B] [--bs 16|32]\n"
-        "                          [--sparsity S] [--iters I] [--copies N]\n"
+        "                          [--sparsity S] [--iters I] [--copies N] [--sym-reps R]\n"
         "       kami_bench spmm    [--prec f16|bf16|f64] [--blocks B] [--bs 16|32] [--n N]\n"
         "                          [--sparsity S]\n");
 }

@@ -282,7 +282,26 @@ def run_w5(steps=10, warmup=2):
     return {"workload": "BSR C=A*A, 8192^2, 32x32 blocks, 2% density, FP16->FP32",
             "nnzb_A": int(len(ci)), "nnzb_C": int(nnz), "nnzCub": nnzcub,
             "symbolic_ms": ms_sym, "numeric_ms": ms_num, "numeric_tflops": flops / ms_num / 1e9,
-            "numeric_hbm_gbs": byts / ms_num / 1e6}
+            "numeric_hbm_gbs": byts / ms_num / 1e6,
+            "symbolic_ms_host_program": symbolic_ms_cli()}
+
+
+def symbolic_ms_cli(blocks=256, bs=32, sparsity=0.98, reps=20):
+    """The symbolic phase timed by the command-line driver, the way the reference's host
+    program times it (count, scan, the total read back, C's column array allocated,
+    fill; SPG-1D:285-370), on the reference's generator at the w5 size and density:
+    milliseconds per call over `reps` calls, or None when the driver is not built."""
+    import os
+    import re
+    import subprocess
+    cli = os.path.join(os.path.dirname(os.path.abspath(__file__)), "kami_bench")
+    if not os.path.exists(cli):
+        return None
+    r = subprocess.run([cli, "spgemm", "--blocks", str(blocks), "--bs", str(bs), "--sparsity",
+                        str(sparsity), "--sym-reps", str(reps)], capture_output=True, text=True,
+                       timeout=300)
+    m = re.search(r"Symbolic time: ([0-9.e+-]+) ms", r.stdout)
+    return float(m.group(1)) if r.returncode == 0 and m else None
 
 
 # ------------------------------------------------------------------ w5r
