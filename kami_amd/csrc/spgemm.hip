@@ -231,6 +231,14 @@ struct SpNumGeom {
     static constexpr int PER_LANE = (CHUNKS + 63) / 64;
 };
 
+// The numeric kernel's workgroup is one wave: its LDS accesses complete in order, so
+// sB needs no workgroup barrier, only this wave's LDS operations finished and a
+// compiler fence.  (__syncthreads() would also wait for every global load in flight.)
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt left at their maxima
+    __builtin_amdgcn_wave_barrier();
+}
+
 __device__ inline int find_col(const int32_t* ci, int lo, int hi, int j) {
     // index in [lo, hi) with ci[idx] == j, or -1 (ci ascending within the row)
     const int end = hi;
@@ -262,6 +270,7 @@ spgemm_numeric_kernel(int mb, int64_t nnzc, const int32_t* rpA, const int32_t* c
     const acc_t alpha = (acc_t)alpha_d;
 
     for (int64_t cb = blockIdx.x; cb < nnzc; cb += gridDim.x) {
+        const int j = ciC[cb];  // issued before the row search, which does not need it
         // block row of output block cb: the last row with rpC[row] <= cb, found by a
         // 64-way search (each round the lanes probe 64 evenly spaced row pointers)
         int lo = 0, hi = mb;
@@ -274,7 +283,7 @@ spgemm_numeric_kernel(int mb, int64_t nnzc, const int32_t* rpA, const int32_t* c
             hi = min(hi, nlo + step);
             lo = nlo;
         }
-        const int i = lo, j = ciC[cb];
+        const int i = lo;
         accv_t acc[G::IM][G::IM];
 #pragma unroll
         for (int a = 0; a < G::IM; ++a)
@@ -301,6 +310,15 @@ spgemm_numeric_kernel(int mb, int64_t nnzc, const int32_t* rpA, const int32_t* c
             const int l = __shfl(lmatch, src);
             const in_t* Ablk = vA + (int64_t)t * BS * BS;
             const in_t* Bblk = vB + (int64_t)l * BS * BS;
+            // A fragments straight to registers, issued with the B block's loads
+            frag_t af[G::KSTEPS][G::IM], bf[G::KSTEPS][G::IM];
+#pragma unroll
+            for (int ks = 0; ks < G::KSTEPS; ++ks) {
+                const int kk = ks * AT::KSTEP + AT::KV * (lane >> 4);
+#pragma unroll
+                for (int x = 0; x < G::IM; ++x)
+                    af[ks][x] = bitcast<frag_t>(load_raw<FB>(Ablk + (x * 16 + (lane & 15)) * BS + kk));
+            }
             // B block (row-major K x N) -> LDS as N rows of K (transposed)
 #pragma unroll
             for (int q = 0; q < G::PER_LANE; ++q) {
@@ -314,21 +332,18 @@ spgemm_numeric_kernel(int mb, int64_t nnzc, const int32_t* rpA, const int32_t* c
                 for (int x = 0; x < 16 / G::ELT; ++x)
                     *reinterpret_cast<in_t*>(sB + (jc + x) * G::ROWB + kr * G::ELT) = e[x];
             }
-            __syncthreads();
+            wave_lds_sync();
             // both blocks' fragments in registers; `iters` > 1 repeats the products on them
             // (the reference's in-kernel NUM_ITER repeat), iters = 1 is the plain product
-            frag_t af[G::KSTEPS][G::IM], bf[G::KSTEPS][G::IM];
 #pragma unroll
             for (int ks = 0; ks < G::KSTEPS; ++ks) {
                 const int kk = ks * AT::KSTEP + AT::KV * (lane >> 4);
 #pragma unroll
-                for (int x = 0; x < G::IM; ++x) {
-                    af[ks][x] = bitcast<frag_t>(load_raw<FB>(Ablk + (x * 16 + (lane & 15)) * BS + kk));
+                for (int x = 0; x < G::IM; ++x)
                     bf[ks][x] = bitcast<frag_t>(
                         load_raw<FB>(sB + (x * 16 + (lane & 15)) * G::ROWB + kk * G::ELT));
-                }
             }
-            __syncthreads();
+            wave_lds_sync();  // the fragments are read before the next match rewrites sB
             for (int it = 0; it < iters; ++it)
 #pragma unroll
                 for (int ks = 0; ks < G::KSTEPS; ++ks)
