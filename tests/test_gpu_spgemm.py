@@ -172,6 +172,24 @@ def test_many_rows_multi_chunk_scan(dev):
     check_structure(A, B, rpA, ciA, rpB, ciB)
 
 
+@pytest.mark.parametrize("mb", [1, 1023, 1024, 4095, 4096, 5000])
+def test_scan_at_chunk_edges_and_repeated_calls(mb, dev):
+    """Row counts on both sides of the scan's 1024-entry chunk edges (the scan covers mb
+    counts and the total) and a single row; each product symbolically computed ten more
+    times: every call gives the oracle's row pointers and total (the pinned total word
+    and its sequence number are reused from call to call)."""
+    rng = np.random.default_rng(8 + mb)
+    rpA, ciA = random_pattern(rng, mb, 40, 0.05)
+    rpB, ciB = random_pattern(rng, 40, 64, 0.1)
+    A = to_bsr(mb, 40, 16, rpA, ciA, np.ones((len(ciA), 16, 16)), dev)
+    B = to_bsr(40, 64, 16, rpB, ciB, np.ones((len(ciB), 16, 16)), dev)
+    rowptr, colidx, rp_ref, nnz_ref = check_structure(A, B, rpA, ciA, rpB, ciB)
+    for _ in range(10):
+        rowptr, nnz = K.spgemm_symbolic(A, B)
+        assert nnz == nnz_ref
+        np.testing.assert_array_equal(rowptr.cpu().numpy(), rp_ref)
+
+
 def test_long_rows_cross_the_pair_chunks(dev):
     """Rows of A with more than 256 blocks and rows whose (A block, B block) pairs number
     several thousand: the symbolic kernels take A's entries 256 at a time and spread the
