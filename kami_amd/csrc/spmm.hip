@@ -360,7 +360,8 @@ __global__ void __launch_bounds__(256) spmm_split_kernel(SpmmArgs a) {
 // The split form serves FP16 / BF16 (64-column tiles, 88 VGPRs: every workgroup of the
 // w6 product resident at once; kernel 10.6 -> 9.2 µs).  FP64 keeps the one-wave-per-16-
 // columns form: its split tiles need 198 VGPRs (two waves per SIMD), and the 2048
-// workgroups then run in four rounds (33.7 -> 39.8 µs).
+// workgroups then run in four rounds (33.7 -> 39.8 µs); with 16-column split tiles (144
+// VGPRs, every column tile loading the row's A blocks again) it ran 42.3-42.6 µs.
 constexpr int kSpmmSplitAtoms = 4;
 
 }  // namespace kami
